@@ -1,0 +1,212 @@
+"""ctypes binding of libp2m.so (include/pks2map_gpu.h).
+
+Python users, the tests and bench.py drive the HIP painter through exactly the
+C ABI the Fortran host uses.  Loading fails loudly if the library is not
+built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libp2m.so")
+
+PREC_PLAIN_FP64 = 0
+PREC_FP64 = 1
+PREC_FP32_COMP = 2
+FLAG_MEMBERSHIP = 1
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "size mismatch", -3: "HIP error", -4: "allocation failed",
+          -5: "radius", -6: "state", -7: "io", -8: "no GPU"}
+
+EXPORTS = ["p2m_create", "p2m_destroy", "p2m_strerror", "p2m_set_run", "p2m_rofz", "p2m_zofr", "p2m_load_table",
+           "p2m_paint", "p2m_paint_device", "p2m_clear", "p2m_fetch_maps", "p2m_fetch_membership", "p2m_get_stats",
+           "p2m_set_tile", "p2m_products", "p2m_write_outputs", "p2m_make_maptable", "p2m_version"]
+
+
+class Config(C.Structure):
+    _fields_ = [("nside", C.c_int32), ("npix_fs", C.c_int32), ("fov_deg", C.c_double), ("model", C.c_int32),
+                ("profile", C.c_int32), ("n_maps", C.c_int32), ("precision", C.c_int32),
+                ("chunk_halos", C.c_int64), ("device", C.c_int32), ("flags", C.c_int32)]
+
+
+class Run(C.Structure):
+    _fields_ = [("omegam", C.c_double), ("omegal", C.c_double), ("h", C.c_double), ("zmin", C.c_double),
+                ("zmax", C.c_double), ("chihview", C.c_double), ("mmin", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_in", C.c_int64), ("n_cut", C.c_int64), ("n_painted", C.c_int64), ("n_listed_hp", C.c_int64),
+                ("n_upd_hp", C.c_int64), ("n_upd_fs", C.c_int64), ("n_fallback", C.c_int64),
+                ("n_segments", C.c_int64), ("n_ring_gaps", C.c_int64), ("n_radius_err", C.c_int64),
+                ("cut_low", C.c_double), ("cut_high", C.c_double), ("ms_total", C.c_double),
+                ("ms_decode", C.c_double), ("ms_bin", C.c_double), ("ms_paint", C.c_double),
+                ("ms_finish", C.c_double), ("tile_w", C.c_int32), ("tile_wg", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load libp2m.so and declare every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m matt_young_mapmaking_amd.build` "
+                           "(there is no CPU fallback for the painter)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    P = C.POINTER
+    L.p2m_create.argtypes = [P(vp), P(Config)]
+    L.p2m_destroy.argtypes = [vp]
+    L.p2m_destroy.restype = None
+    L.p2m_strerror.argtypes = [C.c_int]
+    L.p2m_strerror.restype = C.c_char_p
+    L.p2m_set_run.argtypes = [vp, P(Run)]
+    L.p2m_rofz.argtypes = [vp, d, P(d)]
+    L.p2m_zofr.argtypes = [vp, d, P(d)]
+    L.p2m_load_table.argtypes = [vp, C.c_int, P(i32), P(C.c_float), P(C.c_float), i64]
+    L.p2m_paint.argtypes = [vp, P(C.c_float), i64, i64]
+    L.p2m_paint_device.argtypes = [vp, vp, i64, i64]
+    L.p2m_clear.argtypes = [vp]
+    L.p2m_fetch_maps.argtypes = [vp, C.c_int, P(C.c_float), P(d), P(C.c_float), P(d)]
+    L.p2m_fetch_membership.argtypes = [vp, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32), P(C.c_uint64)]
+    L.p2m_get_stats.argtypes = [vp, P(Stats)]
+    L.p2m_set_tile.argtypes = [vp, i32, i32]
+    L.p2m_products.argtypes = [vp, C.c_int, P(d), P(d), P(C.c_float)]
+    L.p2m_write_outputs.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int]
+    L.p2m_make_maptable.argtypes = [i32, i32, d, d, i32, i32, i32, P(i32), P(C.c_float), P(C.c_float), i64]
+    L.p2m_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+class P2MError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {load().p2m_strerror(code).decode()} ({code})")
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise P2MError(rc, what)
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+class Painter:
+    """One painter context on one GPU (p2m_create ... p2m_destroy)."""
+
+    def __init__(self, nside=0, npix_fs=0, fov_deg=10.0, model=1, profile=1, n_maps=1,
+                 precision=PREC_FP32_COMP, chunk_halos=0, device=0, membership=False):
+        self.L = load()
+        self.cfg = Config(nside, npix_fs, fov_deg, model, profile, n_maps, precision, chunk_halos, device,
+                          FLAG_MEMBERSHIP if membership else 0)
+        self.ctx = C.c_void_p()
+        _chk(self.L.p2m_create(C.byref(self.ctx), C.byref(self.cfg)), "p2m_create")
+
+    def close(self):
+        if self.ctx:
+            self.L.p2m_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_run(self, zmin=0.05, zmax=1.2, chihview=0.0, mmin=1e13, omegam=0.25, omegal=0.75, h=0.7):
+        r = Run(omegam, omegal, h, zmin, zmax, chihview, mmin)
+        _chk(self.L.p2m_set_run(self.ctx, C.byref(r)), "p2m_set_run")
+
+    def rofz(self, z):
+        out = C.c_double()
+        _chk(self.L.p2m_rofz(self.ctx, float(z), C.byref(out)), "p2m_rofz")
+        return out.value
+
+    def zofr(self, chi):
+        out = C.c_double()
+        _chk(self.L.p2m_zofr(self.ctx, float(chi), C.byref(out)), "p2m_zofr")
+        return out.value
+
+    def load_table(self, slot, hdr_i, hdr_f, table):
+        hi = np.ascontiguousarray(hdr_i, np.int32)
+        hf = np.ascontiguousarray(hdr_f, np.float32)
+        t = np.ascontiguousarray(table, np.float32).ravel()
+        _chk(self.L.p2m_load_table(self.ctx, slot, _p(hi, C.c_int32), _p(hf, C.c_float), _p(t, C.c_float), t.size),
+             "p2m_load_table")
+
+    def paint(self, halos, index_base=0):
+        h = np.ascontiguousarray(halos, np.float32).reshape(-1, 7)
+        _chk(self.L.p2m_paint(self.ctx, _p(h, C.c_float), h.shape[0], index_base), "p2m_paint")
+
+    def paint_device(self, dev_ptr, n, index_base=0):
+        _chk(self.L.p2m_paint_device(self.ctx, C.c_void_p(dev_ptr), n, index_base), "p2m_paint_device")
+
+    def clear(self):
+        _chk(self.L.p2m_clear(self.ctx), "p2m_clear")
+
+    def set_tile(self, w, wg):
+        _chk(self.L.p2m_set_tile(self.ctx, w, wg), "p2m_set_tile")
+
+    def maps(self, slot=0, dtype=np.float64):
+        hpn = 12 * self.cfg.nside ** 2 if self.cfg.nside else 0
+        fsn = self.cfg.npix_fs ** 2
+        hp = np.zeros(hpn, dtype) if hpn else None
+        fs = np.zeros(fsn, dtype) if fsn else None
+        if dtype == np.float64:
+            _chk(self.L.p2m_fetch_maps(self.ctx, slot, None, _p(hp, C.c_double), None, _p(fs, C.c_double)), "fetch")
+        else:
+            _chk(self.L.p2m_fetch_maps(self.ctx, slot, _p(hp, C.c_float), None, _p(fs, C.c_float), None), "fetch")
+        return hp, fs
+
+    def membership(self):
+        hpn = 12 * self.cfg.nside ** 2 if self.cfg.nside else 0
+        fsn = self.cfg.npix_fs ** 2
+        hc = np.zeros(hpn, np.uint32) if hpn else None
+        hh = np.zeros(hpn, np.uint64) if hpn else None
+        fc = np.zeros(fsn, np.uint32) if fsn else None
+        fh = np.zeros(fsn, np.uint64) if fsn else None
+        _chk(self.L.p2m_fetch_membership(self.ctx, _p(hc, C.c_uint32), _p(hh, C.c_uint64), _p(fc, C.c_uint32),
+                                         _p(fh, C.c_uint64)), "fetch_membership")
+        return hc, hh, fc, fh
+
+    def stats(self):
+        s = Stats()
+        _chk(self.L.p2m_get_stats(self.ctx, C.byref(s)), "stats")
+        return s.as_dict()
+
+    def products(self, slot=0):
+        mom = np.zeros(6, np.float64)
+        rms = np.zeros(2, np.float64)
+        dp = np.zeros(10000, np.float32)
+        _chk(self.L.p2m_products(self.ctx, slot, _p(mom, C.c_double), _p(rms, C.c_double), _p(dp, C.c_float)),
+             "products")
+        return mom, rms, dp
+
+    def write_outputs(self, base, slot=0, fs=True, hp=True, dp=True):
+        _chk(self.L.p2m_write_outputs(self.ctx, slot, base.encode(), int(fs), int(hp), int(dp)), "write_outputs")
+
+
+def make_maptable(model=1, nu=148.0, alpha=0.0, nchit=100, nmht=100, nrt=200, device=0):
+    L = load()
+    hi = np.zeros(3, np.int32)
+    hf = np.zeros(6, np.float32)
+    t = np.zeros(2 * nchit * nmht * nrt, np.float32)
+    _chk(L.p2m_make_maptable(device, model, nu, alpha, nchit, nmht, nrt, _p(hi, C.c_int32), _p(hf, C.c_float),
+                             _p(t, C.c_float), t.size), "p2m_make_maptable")
+    return hi, hf, t

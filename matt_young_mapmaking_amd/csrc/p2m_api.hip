@@ -1,0 +1,856 @@
+/*
+ * p2m_api.hip -- host runtime behind the C ABI of include/pks2map_gpu.h.
+ *
+ * Owns the device state of one pks2map run: distance tables, profile tables,
+ * maps, per-halo records, tile bins.  A paint call decodes the catalogue in
+ * chunks (appending painted halos to one device array), bins every
+ * (halo, ring) pixel run into ring tiles, and paints all tiles in one launch,
+ * so each map pixel is written once per call.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/pks2map_gpu.h"
+#include "p2m_kernels.h"
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return P2M_E_HIP;     \
+    } while (0)
+
+namespace {
+
+const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
+
+struct Ctr {                 /* layout of the device counter block */
+    enum { N_OUT = 0, DEC = 1, PAINT = 8, TOTAL = 16 };
+};
+
+}  // namespace
+
+struct p2m_ctx {
+    p2m_config cfg;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8];
+    bool have_run = false;
+    bool have_table[P2M_MAXMAPS] = {false};
+    p2m_run run;
+    double cut_low = 0, cut_high = 0, mypi = 0;
+    std::vector<double> rofz, zofr;
+    double *d_rofz = nullptr, *d_zofr = nullptr;
+    CosmoDev cdev;
+    int32_t hdr_i[3] = {0, 0, 0};
+    float hdr_f[6] = {0, 0, 0, 0, 0, 0};
+    TabGeom g;
+    float *d_flat[P2M_MAXMAPS] = {nullptr};
+    float *d_prof[P2M_MAXMAPS] = {nullptr};
+    int64_t n_flat = 0, n_prof = 0;
+    double *d_inv_node2 = nullptr;
+    float fr_clamp = 0.f, u_k1 = 0.f, u_k0 = 0.f, u_max = 0.f;
+    int64_t hpnpix = 0, fsnpix = 0;
+    double *d_hp64[P2M_MAXMAPS] = {nullptr};
+    float *d_hp32[P2M_MAXMAPS] = {nullptr};
+    double *d_fs64[P2M_MAXMAPS] = {nullptr};
+    uint32_t *d_count = nullptr, *d_fs_count = nullptr;
+    unsigned long long *d_hash = nullptr, *d_fs_hash = nullptr;
+    bool hp_zero = true;
+    HaloRec *d_halo = nullptr;
+    int64_t halo_cap = 0;
+    unsigned long long *d_ctr = nullptr;
+    int log2w = 0, wg = 0;
+    bool tuned = false;
+    std::vector<int32_t> tile_base;
+    int32_t *d_tile_base = nullptr;
+    int64_t n_tiles = 0;
+    int n_rings = 0;
+    uint32_t *d_tile_cnt = nullptr, *d_tile_fill = nullptr;
+    unsigned long long *d_tile_off = nullptr;
+    void *d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    uint2 *d_segs = nullptr;
+    int64_t seg_cap = 0;
+    float *d_stage = nullptr;
+    int64_t stage_cap = 0;
+    p2m_stats st;
+};
+
+/* ------------------------------------------------------------------ helpers */
+static int dmalloc(void **p, size_t bytes)
+{
+    if (bytes == 0) { *p = nullptr; return 0; }
+    if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; (void)hipGetLastError(); return P2M_E_NOMEM; }
+    return 0;
+}
+static void dfree(void *p) { if (p) (void)hipFree(p); }
+
+/* cosmology.f90:123-133 (dzdr) and 109-119 (drdz), FP64 */
+static double dzdr(double h, double om, double z)
+{
+    double z1 = 1.0 + z;
+    return ((h * 100.0) / CKMS) * p2m_sqrt((om * ((z1 * z1) * z1) + 1.0) - om);
+}
+
+static double host_lookup(const std::vector<double> &t, double tmin, double dt, double x)
+{
+    if (x != x) return x;
+    int n = (int)t.size();
+    double u = (x - tmin) / dt + 1.0;
+    if (u < 1.0) return t[0];
+    if (u >= (double)n) return t[n - 1];
+    int itab = (int)u;
+    double fint = ((x - (double)(itab - 1) * dt) - tmin) / dt;
+    return (1.0 - fint) * t[itab - 1] + fint * t[itab];
+}
+
+static bool is_pow2(int32_t x) { return x > 0 && (x & (x - 1)) == 0; }
+
+static int build_tiles(p2m_ctx *c, int log2w)
+{
+    int32_t ns = c->cfg.nside;
+    if (ns <= 0) return 0;
+    if (c->log2w == log2w && c->d_tile_base) return 0;
+    int W = 1 << log2w;
+    c->n_rings = 4 * ns - 1;
+    c->tile_base.assign(c->n_rings, 0);
+    int64_t acc = 0;
+    for (int r = 1; r <= c->n_rings; ++r) {
+        int32_t m = std::min(std::min(ns, r), 4 * ns - r);
+        int32_t npr = 4 * m;
+        c->tile_base[r - 1] = (int32_t)acc;
+        acc += (npr + W - 1) / W;
+    }
+    c->n_tiles = acc;
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
+    c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_fill = nullptr; c->d_tile_off = nullptr; c->d_scan_tmp = nullptr;
+    int rc = 0;
+    rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
+    rc |= dmalloc((void **)&c->d_tile_cnt, sizeof(uint32_t) * (c->n_tiles + 1));
+    rc |= dmalloc((void **)&c->d_tile_fill, sizeof(uint32_t) * (c->n_tiles + 1));
+    rc |= dmalloc((void **)&c->d_tile_off, sizeof(unsigned long long) * (c->n_tiles + 1));
+    if (rc) return P2M_E_NOMEM;
+    HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
+    c->scan_tmp_bytes = 0;
+    if (p2m_scan_tiles(c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1, nullptr, &c->scan_tmp_bytes, c->stream)) return P2M_E_HIP;
+    if (dmalloc(&c->d_scan_tmp, c->scan_tmp_bytes + 256)) return P2M_E_NOMEM;
+    c->log2w = log2w;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ ABI */
+extern "C" {
+
+const char *p2m_version(void) { return "p2m 1.0 (gfx950)"; }
+
+const char *p2m_strerror(int code)
+{
+    switch (code) {
+    case P2M_OK: return "ok";
+    case P2M_E_INVALID: return "invalid argument";
+    case P2M_E_SIZE: return "buffer or header size mismatch";
+    case P2M_E_HIP: return "HIP runtime error";
+    case P2M_E_NOMEM: return "allocation failed";
+    case P2M_E_RADIUS: return "disc radius outside [0, pi]";
+    case P2M_E_STATE: return "call out of order (run parameters or table missing)";
+    case P2M_E_IO: return "file could not be written";
+    case P2M_E_NODEVICE: return "no GPU visible";
+    default: return "unknown error";
+    }
+}
+
+int p2m_create(p2m_ctx **out, const p2m_config *cfg)
+{
+    if (!out || !cfg) return P2M_E_INVALID;
+    *out = nullptr;
+    if (cfg->nside < 0 || (cfg->nside > 0 && (!is_pow2(cfg->nside) || cfg->nside > 8192))) return P2M_E_INVALID;
+    if (cfg->npix_fs < 0 || cfg->npix_fs > 65536) return P2M_E_INVALID;
+    if (cfg->n_maps < 1 || cfg->n_maps > P2M_MAXMAPS) return P2M_E_INVALID;
+    if (cfg->model < 1 || cfg->model > 3) return P2M_E_INVALID;
+    if (cfg->profile < 1 || cfg->profile > 2) return P2M_E_INVALID;
+    if (cfg->precision < P2M_PREC_PLAIN_FP64 || cfg->precision > P2M_PREC_FP32_COMP) return P2M_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { (void)hipGetLastError(); return P2M_E_NODEVICE; }
+    if (cfg->device < 0 || cfg->device >= ndev) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(cfg->device));
+    p2m_ctx *c = new p2m_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.chunk_halos <= 0) c->cfg.chunk_halos = 1 << 24;
+    memset(&c->st, 0, sizeof(c->st));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return P2M_E_HIP; }
+    for (int i = 0; i < 8; ++i) (void)hipEventCreate(&c->ev[i]);
+    c->hpnpix = cfg->nside > 0 ? 12LL * cfg->nside * cfg->nside : 0;
+    c->fsnpix = (int64_t)cfg->npix_fs * cfg->npix_fs;
+    int rc = 0;
+    for (int m = 0; m < cfg->n_maps && !rc; ++m) {
+        if (c->hpnpix) {
+            if (cfg->precision == P2M_PREC_FP32_COMP) rc |= dmalloc((void **)&c->d_hp32[m], sizeof(float) * c->hpnpix);
+            else rc |= dmalloc((void **)&c->d_hp64[m], sizeof(double) * c->hpnpix);
+        }
+        if (c->fsnpix) rc |= dmalloc((void **)&c->d_fs64[m], sizeof(double) * c->fsnpix);
+    }
+    if (!rc && (cfg->flags & P2M_FLAG_MEMBERSHIP)) {
+        if (c->hpnpix) {
+            rc |= dmalloc((void **)&c->d_count, sizeof(uint32_t) * c->hpnpix);
+            rc |= dmalloc((void **)&c->d_hash, sizeof(unsigned long long) * c->hpnpix);
+        }
+        if (c->fsnpix) {
+            rc |= dmalloc((void **)&c->d_fs_count, sizeof(uint32_t) * c->fsnpix);
+            rc |= dmalloc((void **)&c->d_fs_hash, sizeof(unsigned long long) * c->fsnpix);
+        }
+    }
+    if (!rc) rc |= dmalloc((void **)&c->d_ctr, sizeof(unsigned long long) * Ctr::TOTAL);
+    if (rc) { p2m_destroy(c); return P2M_E_NOMEM; }
+    *out = c;
+    int e = p2m_clear(c);
+    if (e) { p2m_destroy(c); *out = nullptr; return e; }
+    return 0;
+}
+
+void p2m_destroy(p2m_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int m = 0; m < P2M_MAXMAPS; ++m) {
+        dfree(c->d_hp64[m]); dfree(c->d_hp32[m]); dfree(c->d_fs64[m]); dfree(c->d_flat[m]); dfree(c->d_prof[m]);
+    }
+    dfree(c->d_count); dfree(c->d_hash); dfree(c->d_fs_count); dfree(c->d_fs_hash);
+    dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_halo); dfree(c->d_ctr);
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
+    dfree(c->d_segs); dfree(c->d_stage);
+    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int p2m_clear(p2m_ctx *c)
+{
+    if (!c) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    for (int m = 0; m < c->cfg.n_maps; ++m) {
+        if (c->d_hp64[m]) p2m_launch_fill_f64(c->d_hp64[m], c->hpnpix, c->stream);
+        if (c->d_hp32[m]) p2m_launch_fill_f32(c->d_hp32[m], c->hpnpix, c->stream);
+        if (c->d_fs64[m]) p2m_launch_fill_f64(c->d_fs64[m], c->fsnpix, c->stream);
+    }
+    if (c->d_count) p2m_launch_fill_u32(c->d_count, c->hpnpix, 0u, c->stream);
+    if (c->d_hash) p2m_launch_fill_u64(c->d_hash, c->hpnpix, c->stream);
+    if (c->d_fs_count) p2m_launch_fill_u32(c->d_fs_count, c->fsnpix, 0u, c->stream);
+    if (c->d_fs_hash) p2m_launch_fill_u64(c->d_fs_hash, c->fsnpix, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->hp_zero = true;
+    return 0;
+}
+
+int p2m_set_run(p2m_ctx *c, const p2m_run *r)
+{
+    if (!c || !r) return P2M_E_INVALID;
+    if (!(r->h > 0) || !(r->omegam > 0)) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    c->run = *r;
+    /* set_rofztable, cosmology.f90:51-76 */
+    const int n = 10000;
+    double dz = (6.0 - 0.0) / (double)(n - 1);
+    c->rofz.assign(n, 0.0);
+    for (int i = 2; i <= n; ++i) {
+        double z = (double)(i - 1) * dz + 0.0;
+        c->rofz[i - 1] = c->rofz[i - 2] + (1.0 / dzdr(r->h, r->omegam, z)) * dz;
+    }
+    /* set_zofrtable, cosmology.f90:80-105 */
+    double dr = (1.3e4 - 0.0) / (double)(n - 1);
+    c->zofr.assign(n, 0.0);
+    for (int i = 2; i <= n; ++i) {
+        double z = c->zofr[i - 2];
+        c->zofr[i - 1] = c->zofr[i - 2] + dzdr(r->h, r->omegam, z) * dr;
+    }
+    if (!c->d_rofz && dmalloc((void **)&c->d_rofz, sizeof(double) * n)) return P2M_E_NOMEM;
+    if (!c->d_zofr && dmalloc((void **)&c->d_zofr, sizeof(double) * n)) return P2M_E_NOMEM;
+    HIPCHK(hipMemcpy(c->d_rofz, c->rofz.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_zofr, c->zofr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->mypi = 4.0 * p2m_atan(1.0);                         /* pks2map.f90:88 */
+    c->cdev.rofz = c->d_rofz; c->cdev.zofr = c->d_zofr;
+    c->cdev.drofz = dz; c->cdev.dzofr = dr; c->cdev.rofz_min = 0.0; c->cdev.zofr_min = 0.0;
+    c->cdev.nrofz = n; c->cdev.nzofr = n;
+    c->cdev.omegam = r->omegam; c->cdev.omegal = r->omegal;
+    c->cdev.rho_0 = (2.775e11 * r->omegam) * (r->h * r->h);     /* pks2map.f90:127 */
+    c->cdev.rhocrit_0 = 2.775e11 * (r->h * r->h);               /* :128 */
+    c->cdev.mypi = c->mypi;
+    c->cut_low = host_lookup(c->rofz, 0.0, dz, r->zmin);        /* :134 */
+    c->cut_high = host_lookup(c->rofz, 0.0, dz, r->zmax);       /* :135 */
+    c->have_run = true;
+    return 0;
+}
+
+int p2m_rofz(p2m_ctx *c, double z, double *chi)
+{
+    if (!c || !chi) return P2M_E_INVALID;
+    if (!c->have_run) return P2M_E_STATE;
+    *chi = host_lookup(c->rofz, 0.0, 6.0 / 9999.0, z);
+    return 0;
+}
+
+int p2m_zofr(p2m_ctx *c, double chi, double *z)
+{
+    if (!c || !z) return P2M_E_INVALID;
+    if (!c->have_run) return P2M_E_STATE;
+    *z = host_lookup(c->zofr, 0.0, 1.3e4 / 9999.0, chi);
+    return 0;
+}
+
+int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr_f[6], const float *table, int64_t n_table)
+{
+    if (!c || !hdr_i || !hdr_f || !table) return P2M_E_INVALID;
+    if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
+    if (hdr_i[0] < 2 || hdr_i[1] < 2 || hdr_i[2] < 2) return P2M_E_SIZE;
+    int64_t nt = 2LL * hdr_i[0] * hdr_i[1] * hdr_i[2];
+    if (n_table != nt) return P2M_E_SIZE;
+    if (!(hdr_f[0] > 0 && hdr_f[1] > hdr_f[0] && hdr_f[2] > 0 && hdr_f[3] > hdr_f[2] && hdr_f[4] > 0 && hdr_f[5] > hdr_f[4]))
+        return P2M_E_INVALID;
+    bool any = false;
+    for (int m = 0; m < c->cfg.n_maps; ++m) any |= c->have_table[m];
+    if (any && (memcmp(hdr_i, c->hdr_i, sizeof(c->hdr_i)) || memcmp(hdr_f, c->hdr_f, sizeof(c->hdr_f))))
+        return P2M_E_SIZE;   /* all slots share one grid */
+    HIPCHK(hipSetDevice(c->cfg.device));
+    memcpy(c->hdr_i, hdr_i, sizeof(c->hdr_i));
+    memcpy(c->hdr_f, hdr_f, sizeof(c->hdr_f));
+    TabGeom &g = c->g;
+    g.nchit = hdr_i[0]; g.nmht = hdr_i[1]; g.nrt = hdr_i[2];
+    g.chimint = hdr_f[0]; g.chimaxt = hdr_f[1]; g.mhmint = hdr_f[2]; g.mhmaxt = hdr_f[3];
+    g.rmint = hdr_f[4]; g.rmaxt = hdr_f[5];
+    /* maptable.f90:166-168 */
+    g.dchit = (p2m_log(g.chimaxt) - p2m_log(g.chimint)) / (double)(g.nchit - 1);
+    g.drt = (p2m_log(g.rmaxt) - p2m_log(g.rmint)) / (double)(g.nrt - 1);
+    g.dmht = (p2m_log(g.mhmaxt) - p2m_log(g.mhmint)) / (double)(g.nmht - 1);
+    g.lr0 = p2m_log(g.rmint); g.lc0 = p2m_log(g.chimint); g.lm0 = p2m_log(g.mhmint);
+    /* padding covers every index interpolate_table can form past the end (Q6) */
+    int64_t pad = 2LL * g.nrt * (g.nmht + 2) + 4LL * g.nrt + 64;
+    c->n_flat = nt + pad;
+    c->n_prof = nt / 2 + pad / 2;
+    if (!c->d_flat[slot] && dmalloc((void **)&c->d_flat[slot], sizeof(float) * c->n_flat)) return P2M_E_NOMEM;
+    if (!c->d_prof[slot] && dmalloc((void **)&c->d_prof[slot], sizeof(float) * c->n_prof)) return P2M_E_NOMEM;
+    std::vector<float> flat(c->n_flat, 0.f), prof(c->n_prof, 0.f);
+    memcpy(flat.data(), table, sizeof(float) * nt);
+    int p = c->cfg.profile;
+    for (int64_t i = 0; i < nt / 2; ++i) prof[i] = table[(p - 1) + 2 * i];
+    HIPCHK(hipMemcpy(c->d_flat[slot], flat.data(), sizeof(float) * c->n_flat, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_prof[slot], prof.data(), sizeof(float) * c->n_prof, hipMemcpyHostToDevice));
+    /* FP32 path constants: node radii, ln r -> cell coordinate */
+    std::vector<double> inv(g.nrt + 2);
+    for (int k = 0; k < g.nrt + 2; ++k) inv[k] = p2m_exp(-2.0 * (g.lr0 + (double)k * g.drt));
+    if (!c->d_inv_node2 && dmalloc((void **)&c->d_inv_node2, sizeof(double) * (g.nrt + 2))) return P2M_E_NOMEM;
+    HIPCHK(hipMemcpy(c->d_inv_node2, inv.data(), sizeof(double) * (g.nrt + 2), hipMemcpyHostToDevice));
+    c->u_k1 = (float)(0.5 * 0.6931471805599453 / g.drt);   /* log2(r^2) -> (ln r)/drt */
+    c->u_k0 = (float)(g.lr0 / g.drt);
+    c->u_max = (float)(g.nrt - 1);
+    c->fr_clamp = (float)(p2m_log(g.rmint + 1e-5) - g.lr0);
+    c->have_table[slot] = true;
+    return 0;
+}
+
+int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
+{
+    if (!c) return P2M_E_INVALID;
+    if (tile_w == 0 && tile_wg == 0) { c->tuned = false; return 0; }
+    if (!is_pow2(tile_w) || tile_w < 64 || tile_w > 4096) return P2M_E_INVALID;
+    if (tile_wg < 64 || tile_wg > 1024 || tile_wg % 64) return P2M_E_INVALID;
+    int l = 0;
+    while ((1 << l) < tile_w) ++l;
+    if (p2m_tile_lds_bytes(c->cfg.precision == P2M_PREC_FP64 ? 1 : 2, c->cfg.n_maps, l, c->d_count != nullptr) > 160 * 1024)
+        return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc = build_tiles(c, l);
+    if (rc) return rc;
+    c->wg = tile_wg;
+    c->tuned = true;
+    return 0;
+}
+
+}  /* extern "C" */
+
+/* ------------------------------------------------------------------ paint */
+static int ensure_halo_cap(p2m_ctx *c, int64_t n)
+{
+    if (n <= c->halo_cap) return 0;
+    dfree(c->d_halo); c->d_halo = nullptr; c->halo_cap = 0;
+    int64_t cap = std::max<int64_t>(n, 1024);
+    if (dmalloc((void **)&c->d_halo, sizeof(HaloRec) * cap)) return P2M_E_NOMEM;
+    c->halo_cap = cap;
+    return 0;
+}
+
+static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
+{
+    DecodeArgs a;
+    a.halos = d_src; a.n = n; a.index_base = base;
+    a.c = c->cdev; a.g = c->g;
+    a.cut_low = c->cut_low; a.cut_high = c->cut_high; a.chihview = c->run.chihview; a.mmin = c->run.mmin;
+    a.dmodel = c->cfg.model == 2 ? 500.0 : 200.0;
+    a.profile = c->cfg.profile; a.nside = c->cfg.nside; a.npix_fs = c->cfg.npix_fs;
+    a.fov = ((c->cfg.fov_deg / 360.0) * 2.0) * c->mypi;        /* pks2map.f90:92 */
+    a.dpix = c->cfg.npix_fs > 0 ? a.fov / (double)c->cfg.npix_fs : 1.0;
+    a.out = c->d_halo; a.cap = c->halo_cap;
+    a.n_out = c->d_ctr + Ctr::N_OUT;
+    a.counters = c->d_ctr + Ctr::DEC;
+    p2m_launch_decode(a, c->stream);
+    return hipGetLastError() == hipSuccess ? 0 : P2M_E_HIP;
+}
+
+static int autotune(p2m_ctx *c, int64_t n_halo);
+
+static int paint_halos(p2m_ctx *c, int64_t n_halo, bool allow_tune)
+{
+    const int mode = c->cfg.precision;
+    unsigned long long *pc = c->d_ctr + Ctr::PAINT;
+    PaintArgs pa;
+    memset(&pa, 0, sizeof(pa));
+    pa.halos = c->d_halo; pa.n_halo = n_halo; pa.nside = c->cfg.nside; pa.npix_fs = c->cfg.npix_fs;
+    pa.n_maps = c->cfg.n_maps; pa.profile = c->cfg.profile;
+    pa.fov = ((c->cfg.fov_deg / 360.0) * 2.0) * c->mypi;
+    pa.dpix = c->cfg.npix_fs > 0 ? pa.fov / (double)c->cfg.npix_fs : 1.0;
+    pa.g = c->g; pa.n_flat = c->n_flat;
+    for (int m = 0; m < c->cfg.n_maps; ++m) { pa.tab_flat[m] = c->d_flat[m]; pa.map64[m] = c->d_hp64[m]; pa.fs64[m] = c->d_fs64[m]; }
+    pa.count = c->d_count; pa.hash = c->d_hash; pa.fs_count = c->d_fs_count; pa.fs_hash = c->d_fs_hash;
+    pa.counters = pc;
+
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    if (mode == P2M_PREC_PLAIN_FP64) {
+        if (c->cfg.nside > 0) p2m_launch_paint_plain_hp(pa, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[3], c->stream));
+        if (c->cfg.npix_fs > 0) p2m_launch_paint_fs(pa, 1, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(hipEventRecord(c->ev[5], c->stream));
+        c->hp_zero = false;
+        return 0;
+    }
+    if (c->cfg.nside > 0) {
+        if (!c->tuned) {
+            if (allow_tune) { int rc = autotune(c, n_halo); if (rc) return rc; }
+            else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
+        }
+        BinArgs b;
+        b.halos = c->d_halo; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
+        b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt; b.tile_fill = c->d_tile_fill;
+        b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc;
+        p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
+        p2m_launch_fill_u32(c->d_tile_fill, c->n_tiles + 1, 0u, c->stream);
+        p2m_launch_seg_count(b, c->stream);
+        if (p2m_scan_tiles(c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1, c->d_scan_tmp, &c->scan_tmp_bytes, c->stream))
+            return P2M_E_HIP;
+        unsigned long long nseg = 0;
+        HIPCHK(hipMemcpyAsync(&nseg, c->d_tile_off + c->n_tiles, sizeof(nseg), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if ((int64_t)nseg > c->seg_cap) {
+            dfree(c->d_segs); c->d_segs = nullptr; c->seg_cap = 0;
+            int64_t cap = (int64_t)(nseg * 1.25) + 1024;
+            if (dmalloc((void **)&c->d_segs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
+            c->seg_cap = cap;
+        }
+        b.segs = c->d_segs;
+        p2m_launch_seg_emit(b, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[3], c->stream));
+        TileArgs t;
+        memset(&t, 0, sizeof(t));
+        t.halos = c->d_halo; t.nside = c->cfg.nside; t.log2w = c->log2w; t.n_rings = c->n_rings;
+        t.n_maps = c->cfg.n_maps; t.profile = c->cfg.profile; t.accumulate = c->hp_zero ? 0 : 1;
+        t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
+        t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
+        t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
+        for (int m = 0; m < c->cfg.n_maps; ++m) {
+            t.tab_flat[m] = c->d_flat[m]; t.tab_prof[m] = c->d_prof[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
+        }
+        t.count = c->d_count; t.hash = c->d_hash; t.counters = pc;
+        int grid = (int)std::min<int64_t>(c->n_tiles, 1 << 30);
+        if (p2m_launch_tile_paint(t, mode, c->cfg.n_maps, grid, c->wg, c->stream)) return P2M_E_INVALID;
+        HIPCHK(hipGetLastError());
+    } else {
+        HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    if (c->cfg.npix_fs > 0) p2m_launch_paint_fs(pa, 0, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    c->hp_zero = false;
+    return 0;
+}
+
+static int finish_stats(p2m_ctx *c)
+{
+    unsigned long long h[Ctr::TOTAL];
+    HIPCHK(hipMemcpyAsync(h, c->d_ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    p2m_stats &s = c->st;
+    s.n_cut = (int64_t)h[Ctr::DEC + 0];
+    s.n_painted = (int64_t)h[Ctr::DEC + 1];
+    s.n_radius_err = (int64_t)h[Ctr::DEC + 2];
+    s.n_listed_hp = (int64_t)h[Ctr::PAINT + 0];
+    s.n_upd_hp = (int64_t)h[Ctr::PAINT + 1];
+    s.n_upd_fs = (int64_t)h[Ctr::PAINT + 2];
+    s.n_ring_gaps = (int64_t)h[Ctr::PAINT + 3];
+    s.n_segments = (int64_t)h[Ctr::PAINT + 4];
+    s.n_fallback = (int64_t)h[Ctr::PAINT + 5];
+    s.cut_low = c->cut_low; s.cut_high = c->cut_high;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); s.ms_total = ms;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); s.ms_decode = ms;
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]); s.ms_bin = ms;
+    (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); s.ms_paint = ms;
+    (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]); s.ms_finish = ms;
+    s.tile_w = c->log2w ? (1 << c->log2w) : 0;
+    s.tile_wg = c->wg;
+    return 0;
+}
+
+static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n, int64_t index_base, bool allow_tune)
+{
+    if (!c || (!src && n > 0) || n < 0) return P2M_E_INVALID;
+    if (!c->have_run) return P2M_E_STATE;
+    for (int m = 0; m < c->cfg.n_maps; ++m) if (!c->have_table[m]) return P2M_E_STATE;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc = ensure_halo_cap(c, n);
+    if (rc) return rc;
+    p2m_launch_fill_u64(c->d_ctr, Ctr::TOTAL, c->stream);
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    const int64_t chunk = c->cfg.chunk_halos;
+    if (!on_device && n > 0) {
+        int64_t need = std::min(chunk, n) * 7;
+        if (need > c->stage_cap) {
+            dfree(c->d_stage); c->d_stage = nullptr; c->stage_cap = 0;
+            if (dmalloc((void **)&c->d_stage, sizeof(float) * need)) return P2M_E_NOMEM;
+            c->stage_cap = need;
+        }
+    }
+    for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+        int64_t nc = std::min(chunk, n - c0);
+        const float *d = src + 7 * c0;
+        if (!on_device) {
+            HIPCHK(hipMemcpyAsync(c->d_stage, d, sizeof(float) * 7 * nc, hipMemcpyHostToDevice, c->stream));
+            d = c->d_stage;
+        }
+        rc = decode_chunk(c, d, nc, index_base + c0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    unsigned long long n_out = 0;
+    HIPCHK(hipMemcpyAsync(&n_out, c->d_ctr + Ctr::N_OUT, sizeof(n_out), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int64_t n_halo = (int64_t)std::min<unsigned long long>(n_out, (unsigned long long)c->halo_cap);
+    rc = paint_halos(c, n_halo, allow_tune);
+    if (rc) return rc;
+    c->st.n_in = n;
+    return finish_stats(c);
+}
+
+extern "C" {
+
+int p2m_paint(p2m_ctx *c, const float *halos, int64_t n, int64_t index_base)
+{
+    return paint_common(c, halos, false, n, index_base, true);
+}
+
+int p2m_paint_device(p2m_ctx *c, const float *d_halos, int64_t n, int64_t index_base)
+{
+    return paint_common(c, d_halos, true, n, index_base, true);
+}
+
+int p2m_get_stats(p2m_ctx *c, p2m_stats *st)
+{
+    if (!c || !st) return P2M_E_INVALID;
+    *st = c->st;
+    return 0;
+}
+
+int p2m_fetch_maps(p2m_ctx *c, int slot, float *hp32, double *hp64, float *fs32, double *fs64)
+{
+    if (!c) return P2M_E_INVALID;
+    if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->hpnpix > 0) {
+        if (c->d_hp64[slot]) {
+            if (hp64) HIPCHK(hipMemcpy(hp64, c->d_hp64[slot], sizeof(double) * c->hpnpix, hipMemcpyDeviceToHost));
+            if (hp32) {
+                std::vector<double> tmp(c->hpnpix);
+                HIPCHK(hipMemcpy(tmp.data(), c->d_hp64[slot], sizeof(double) * c->hpnpix, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < c->hpnpix; ++i) hp32[i] = (float)tmp[i];
+            }
+        } else {
+            if (hp32) HIPCHK(hipMemcpy(hp32, c->d_hp32[slot], sizeof(float) * c->hpnpix, hipMemcpyDeviceToHost));
+            if (hp64) {
+                std::vector<float> tmp(c->hpnpix);
+                HIPCHK(hipMemcpy(tmp.data(), c->d_hp32[slot], sizeof(float) * c->hpnpix, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < c->hpnpix; ++i) hp64[i] = (double)tmp[i];
+            }
+        }
+    }
+    if (c->fsnpix > 0 && (fs32 || fs64)) {
+        std::vector<double> tmp(c->fsnpix);
+        HIPCHK(hipMemcpy(tmp.data(), c->d_fs64[slot], sizeof(double) * c->fsnpix, hipMemcpyDeviceToHost));
+        if (fs64) memcpy(fs64, tmp.data(), sizeof(double) * c->fsnpix);
+        if (fs32) for (int64_t i = 0; i < c->fsnpix; ++i) fs32[i] = (float)tmp[i];
+    }
+    return 0;
+}
+
+int p2m_fetch_membership(p2m_ctx *c, uint32_t *hc, uint64_t *hh, uint32_t *fc, uint64_t *fh)
+{
+    if (!c) return P2M_E_INVALID;
+    if (!(c->cfg.flags & P2M_FLAG_MEMBERSHIP)) return P2M_E_STATE;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->hpnpix > 0) {
+        if (hc) HIPCHK(hipMemcpy(hc, c->d_count, sizeof(uint32_t) * c->hpnpix, hipMemcpyDeviceToHost));
+        if (hh) HIPCHK(hipMemcpy(hh, c->d_hash, sizeof(uint64_t) * c->hpnpix, hipMemcpyDeviceToHost));
+    }
+    if (c->fsnpix > 0) {
+        if (fc) HIPCHK(hipMemcpy(fc, c->d_fs_count, sizeof(uint32_t) * c->fsnpix, hipMemcpyDeviceToHost));
+        if (fh) HIPCHK(hipMemcpy(fh, c->d_fs_hash, sizeof(uint64_t) * c->fsnpix, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+}  /* extern "C" */
+
+/* ------------------------------------------------------------------ autotuner */
+/* Times the binning + tile paint for each candidate tile shape on the
+ * halos already decoded, keeps the fastest, and restores the maps. */
+static int autotune(p2m_ctx *c, int64_t n_halo)
+{
+    static const int cand[][2] = {{9, 256}, {10, 256}, {10, 512}, {11, 256}, {11, 512}};
+    int mode_ld = c->cfg.precision == P2M_PREC_FP64 ? 1 : 2;
+    int best_l = 10, best_wg = 256;
+    /* small problems: the default shape */
+    if (n_halo < 200000 || getenv("P2M_NO_AUTOTUNE")) {
+        int rc = build_tiles(c, best_l); if (rc) return rc;
+        c->wg = best_wg; c->tuned = true;
+        return 0;
+    }
+    /* tune on a copy of the maps' state: paint into the maps, then clear
+     * (the call that triggers tuning starts from the maps' current state,
+     * so remember whether they were zero and re-zero after each trial) */
+    bool was_zero = c->hp_zero;
+    if (!was_zero) {
+        int rc = build_tiles(c, best_l); if (rc) return rc;
+        c->wg = best_wg; c->tuned = true;
+        return 0;
+    }
+    float best_ms = 1e30f;
+    for (auto &cd : cand) {
+        if (p2m_tile_lds_bytes(mode_ld, c->cfg.n_maps, cd[0], c->d_count != nullptr) > 160 * 1024) continue;
+        int rc = build_tiles(c, cd[0]); if (rc) return rc;
+        c->wg = cd[1]; c->tuned = true;
+        c->hp_zero = true;
+        rc = paint_halos(c, n_halo, false); if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[4]);
+        if (ms < best_ms) { best_ms = ms; best_l = cd[0]; best_wg = cd[1]; }
+        rc = p2m_clear(c); if (rc) return rc;
+    }
+    /* counters were polluted by the trials: reset the paint counters */
+    p2m_launch_fill_u64(c->d_ctr + Ctr::PAINT, Ctr::TOTAL - Ctr::PAINT, c->stream);
+    int rc = build_tiles(c, best_l); if (rc) return rc;
+    c->wg = best_wg; c->tuned = true; c->hp_zero = true;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ products + writers */
+extern "C" {
+
+int p2m_products(p2m_ctx *c, int slot, double moments[6], double rms[2], float dpdy[10000])
+{
+    if (!c) return P2M_E_INVALID;
+    if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    double *d_out = nullptr;
+    unsigned long long *d_hist = nullptr;
+    if (dmalloc((void **)&d_out, sizeof(double) * 8)) return P2M_E_NOMEM;
+    if (dmalloc((void **)&d_hist, sizeof(unsigned long long) * 10000)) { dfree(d_out); return P2M_E_NOMEM; }
+    double res[2][4] = {{0}};
+    const int ndpdy = 10000;
+    const float dy = 1e-4f / (float)ndpdy;                 /* pks2map.f90:39-40 */
+    int rc = 0;
+    for (int which = 0; which < 2 && !rc; ++which) {
+        int64_t n = which == 0 ? c->fsnpix : c->hpnpix;
+        if (n <= 0) continue;
+        const float *m32 = which == 1 ? c->d_hp32[slot] : nullptr;
+        const double *m64 = which == 1 ? c->d_hp64[slot] : c->d_fs64[slot];
+        if (m32) m64 = nullptr;
+        p2m_launch_fill_f64(d_out, 8, c->stream);
+        if (which == 1) p2m_launch_fill_u64(d_hist, ndpdy, c->stream);
+        p2m_launch_moments(m32, m64, n, 0.0, d_out, which == 1 ? d_hist : nullptr, ndpdy, dy, c->stream);
+        double h1[8];
+        if (hipMemcpyAsync(h1, d_out, sizeof(h1), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = P2M_E_HIP; break; }
+        double mean = h1[0] / (double)n;
+        p2m_launch_fill_f64(d_out, 8, c->stream);
+        p2m_launch_moments(m32, m64, n, mean, d_out, nullptr, ndpdy, dy, c->stream);
+        double h2[8];
+        if (hipMemcpyAsync(h2, d_out, sizeof(h2), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = P2M_E_HIP; break; }
+        res[which][0] = h1[0] / (double)n; res[which][1] = h1[1] / (double)n; res[which][2] = h1[2] / (double)n;
+        res[which][3] = p2m_sqrt(h2[3] / (double)n);
+        if (which == 1 && dpdy) {
+            std::vector<unsigned long long> hist(ndpdy);
+            if (hipMemcpy(hist.data(), d_hist, sizeof(unsigned long long) * ndpdy, hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = P2M_E_HIP; break;
+            }
+            /* dpdy = dpdy / real(hpnpix); cumulative (pks2map.f90:347, 355-357) */
+            for (int i = 0; i < ndpdy; ++i) dpdy[i] = (float)hist[i] / (float)n;
+            for (int i = 1; i < ndpdy; ++i) dpdy[i] = dpdy[i - 1] + dpdy[i];
+        }
+    }
+    dfree(d_out); dfree(d_hist);
+    if (rc) return rc;
+    if (moments) { for (int k = 0; k < 3; ++k) { moments[k] = res[0][k]; moments[3 + k] = res[1][k]; } }
+    if (rms) { rms[0] = res[0][3]; rms[1] = res[1][3]; }
+    return 0;
+}
+
+}  /* extern "C" */
+
+/* FITS card helpers (FITS standard 4.0, sec. 4.2) */
+static std::string card(const char *key, const std::string &val, const char *comment)
+{
+    char b[81];
+    std::string s;
+    if (val.empty()) {
+        snprintf(b, sizeof(b), "%-80s", key);
+        return std::string(b, 80);
+    }
+    snprintf(b, sizeof(b), "%-8s= %20s / %s", key, val.c_str(), comment ? comment : "");
+    s = b;
+    s.resize(80, ' ');
+    return s;
+}
+static std::string sval(const char *v)
+{
+    std::string s = "'";
+    s += v;
+    while (s.size() < 9) s += ' ';
+    s += "'";
+    while (s.size() < 20) s += ' ';
+    return s;
+}
+static std::string ival(long long v) { char b[32]; snprintf(b, sizeof(b), "%lld", v); return b; }
+
+static bool write_block(FILE *f, const std::vector<std::string> &cards)
+{
+    std::string txt;
+    for (auto &c : cards) txt += c;
+    std::string end = "END";
+    end.resize(80, ' ');
+    txt += end;
+    txt.resize(((txt.size() + 2879) / 2880) * 2880, ' ');
+    return fwrite(txt.data(), 1, txt.size(), f) == txt.size();
+}
+
+/* HEALPix map as FITS BINTABLE (fits_s_inc.f90:787-960 via cfitsio in the
+ * reference; the same HDU structure and keywords written directly). */
+static int write_fits(const char *path, const float *m, int64_t npix, int32_t nside)
+{
+    int64_t repeat = 1024;
+    if (npix < 1024) repeat = 1;
+    if (nside > 1024 * 256) repeat = nside / 256;
+    int64_t nrows = npix / repeat;
+    FILE *f = fopen(path, "wb");
+    if (!f) return P2M_E_IO;
+    char date[32];
+    time_t t = time(nullptr);
+    struct tm tmv;
+    gmtime_r(&t, &tmv);
+    strftime(date, sizeof(date), "%Y-%m-%d", &tmv);
+    std::vector<std::string> prim = {
+        card("SIMPLE", "T", "file does conform to FITS standard"),
+        card("BITPIX", ival(32), "number of bits per data pixel"),
+        card("NAXIS", ival(0), "number of data axes"),
+        card("EXTEND", "T", "FITS dataset may contain extensions"),
+        card("DATE", sval(date), "file creation date (YYYY-MM-DD)")};
+    char tform[32];
+    snprintf(tform, sizeof(tform), "%lldE", (long long)repeat);
+    std::vector<std::string> ext = {
+        card("XTENSION", sval("BINTABLE"), "binary table extension"),
+        card("BITPIX", ival(8), "8-bit bytes"),
+        card("NAXIS", ival(2), "2-dimensional binary table"),
+        card("NAXIS1", ival(4 * repeat), "width of table in bytes"),
+        card("NAXIS2", ival(nrows), "number of rows in table"),
+        card("PCOUNT", ival(0), "size of special data area"),
+        card("GCOUNT", ival(1), "one data group (required keyword)"),
+        card("TFIELDS", ival(1), "number of fields in each row"),
+        card("TTYPE1", sval("simulation"), "label for field   1"),
+        card("TFORM1", sval(tform), "data format of field: 4-byte REAL"),
+        card("NSIDE", ival(nside), "the nside of the map"),
+        card("ORDERING", sval("RING"), "the nside of the map")};
+    bool ok = write_block(f, prim) && write_block(f, ext);
+    std::vector<uint32_t> buf(1 << 20);
+    for (int64_t i = 0; ok && i < npix; i += (int64_t)buf.size()) {
+        int64_t k = std::min<int64_t>((int64_t)buf.size(), npix - i);
+        for (int64_t j = 0; j < k; ++j) {
+            uint32_t u;
+            memcpy(&u, &m[i + j], 4);
+            buf[j] = __builtin_bswap32(u);
+        }
+        ok = fwrite(buf.data(), 4, (size_t)k, f) == (size_t)k;
+    }
+    int64_t bytes = npix * 4, pad = (2880 - bytes % 2880) % 2880;
+    std::vector<char> z(pad, 0);
+    if (ok && pad) ok = fwrite(z.data(), 1, pad, f) == (size_t)pad;
+    if (fclose(f) != 0) ok = false;
+    if (!ok) { remove(path); return P2M_E_IO; }
+    return 0;
+}
+
+extern "C" int p2m_write_outputs(p2m_ctx *c, int slot, const char *base, int write_fs, int write_hp, int write_dp)
+{
+    if (!c || !base) return P2M_E_INVALID;
+    if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
+    std::string b(base);
+    int rc = 0;
+    if (write_dp && c->hpnpix > 0) {
+        std::vector<float> dpdy(10000);
+        rc = p2m_products(c, slot, nullptr, nullptr, dpdy.data());
+        if (rc) return rc;
+        std::string p = b + "_dp.bin";
+        FILE *f = fopen(p.c_str(), "wb");
+        if (!f) return P2M_E_IO;
+        int32_t nd = 10000;
+        float mx = 1e-4f;
+        bool ok = fwrite(&nd, 4, 1, f) == 1 && fwrite(&mx, 4, 1, f) == 1 && fwrite(dpdy.data(), 4, 10000, f) == 10000;
+        if (fclose(f) != 0) ok = false;
+        if (!ok) { remove(p.c_str()); return P2M_E_IO; }
+    }
+    if (write_fs && c->fsnpix > 0) {
+        std::vector<float> m(c->fsnpix);
+        rc = p2m_fetch_maps(c, slot, nullptr, nullptr, m.data(), nullptr);
+        if (rc) return rc;
+        std::string p = b + "_fs.map";
+        FILE *f = fopen(p.c_str(), "wb");
+        if (!f) return P2M_E_IO;
+        int32_t np = c->cfg.npix_fs;
+        float fov = (float)(((c->cfg.fov_deg / 360.0) * 2.0) * c->mypi);
+        bool ok = fwrite(&np, 4, 1, f) == 1 && fwrite(&np, 4, 1, f) == 1 && fwrite(&fov, 4, 1, f) == 1 &&
+                  fwrite(&fov, 4, 1, f) == 1 && fwrite(m.data(), 4, (size_t)c->fsnpix, f) == (size_t)c->fsnpix;
+        if (fclose(f) != 0) ok = false;
+        if (!ok) { remove(p.c_str()); return P2M_E_IO; }
+    }
+    if (write_hp && c->hpnpix > 0) {
+        std::vector<float> m(c->hpnpix);
+        rc = p2m_fetch_maps(c, slot, m.data(), nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        std::string p = b + "_hp.fits";
+        rc = write_fits(p.c_str(), m.data(), c->hpnpix, c->cfg.nside);
+        if (rc) return rc;
+    }
+    return 0;
+}

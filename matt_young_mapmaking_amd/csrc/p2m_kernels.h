@@ -1,0 +1,96 @@
+/*
+ * p2m_kernels.h -- launch interface between the host runtime (p2m_api.hip)
+ * and the gfx950 kernels (p2m_kernels.hip).
+ */
+#ifndef P2M_KERNELS_H
+#define P2M_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "p2m_dev.h"
+
+struct DecodeArgs {
+    const float *halos;          /* 7 float32 per halo */
+    int64_t n, index_base;
+    CosmoDev c;
+    TabGeom g;
+    double cut_low, cut_high, chihview, mmin, dmodel;
+    int32_t profile, nside, npix_fs;
+    double fov, dpix;
+    HaloRec *out;
+    int64_t cap;
+    unsigned long long *n_out;   /* append counter */
+    unsigned long long *counters;/* [0] passed radial cut, [1] painted, [2] radius errors */
+};
+
+struct PaintArgs {
+    const HaloRec *halos;
+    int64_t n_halo;
+    int32_t nside, npix_fs, n_maps, profile;
+    double fov, dpix;
+    TabGeom g;
+    const float *tab_flat[P2M_MAXMAPS];
+    int64_t n_flat;
+    double *map64[P2M_MAXMAPS];
+    double *fs64[P2M_MAXMAPS];
+    uint32_t *count;
+    unsigned long long *hash;
+    uint32_t *fs_count;
+    unsigned long long *fs_hash;
+    unsigned long long *counters; /* [0] listed, [1] hp updates, [2] fs updates, [3] ring gaps */
+};
+
+struct BinArgs {
+    const HaloRec *halos;
+    int64_t n_halo;
+    int32_t nside, log2w;
+    const int32_t *tile_base;     /* first tile of ring r+1 */
+    uint32_t *tile_cnt;
+    uint32_t *tile_fill;
+    const unsigned long long *tile_off;
+    uint2 *segs;                  /* (halo, lo<<16 | len) */
+    unsigned long long *counters; /* [0] listed, [3] ring gaps, [4] segments */
+};
+
+struct TileArgs {
+    const HaloRec *halos;
+    int32_t nside, log2w, n_rings, n_maps, profile, accumulate;
+    int64_t n_tiles;
+    const int32_t *tile_base;
+    const unsigned long long *tile_off;
+    const uint2 *segs;
+    TabGeom g;
+    const float *tab_flat[P2M_MAXMAPS];
+    const float *tab_prof[P2M_MAXMAPS];
+    int64_t n_flat;
+    const double *inv_node2;      /* exp(-2 (lr0 + k drt)), k = 0..nrt */
+    float u_k1, u_k0, u_max, band, fr_clamp;
+    double *map64[P2M_MAXMAPS];
+    float *map32[P2M_MAXMAPS];
+    uint32_t *count;
+    unsigned long long *hash;
+    unsigned long long *counters; /* [1] hp updates, [5] FP64 fallbacks */
+};
+
+void p2m_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
+void p2m_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
+void p2m_launch_fill_f64(double *p, int64_t n, hipStream_t s);
+void p2m_launch_fill_f32(float *p, int64_t n, hipStream_t s);
+void p2m_launch_decode(const DecodeArgs &a, hipStream_t s);
+void p2m_launch_paint_plain_hp(const PaintArgs &a, hipStream_t s);
+void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s);
+void p2m_launch_seg_count(const BinArgs &b, hipStream_t s);
+void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
+int p2m_scan_tiles(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
+size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership);
+int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);
+void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double mean, double *out,
+                        unsigned long long *hist, int nb, float dy, hipStream_t s);
+void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s);
+void p2m_launch_f32_to_f64(const float *in, double *out, int64_t n, hipStream_t s);
+
+/* make_maptable kernel (p2m_maptable.hip) */
+int p2m_launch_make_maptable(int32_t model, double nu, double alpha_cmb, int32_t nchit, int32_t nmht, int32_t nrt,
+                             const double *d_rofz, const double *d_zofr, float *d_table, hipStream_t s);
+
+#endif

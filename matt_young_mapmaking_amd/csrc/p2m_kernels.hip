@@ -1,0 +1,731 @@
+/*
+ * p2m_kernels.hip -- gfx950 kernels of the halo->map painter.
+ *
+ *  k_decode          catalogue decode + cull + FP64 per-halo setup
+ *                    (pks2map.f90:144-160, 264, 284-298; haloproject.f90:47-55,
+ *                    93-128; maptable.f90:184-202; query_disc prologue)
+ *  k_paint_plain_hp  baseline: one work item per halo, naive query_disc loop,
+ *                    pix2vec_ring/angdist/interpolate_table per pixel, FP64
+ *                    global atomics (haloproject.f90:55-75)
+ *  k_paint_fs        flat-sky painter, one wave per halo (haloproject.f90:136-152)
+ *  k_seg_count/emit  bin the (halo, ring) pixel runs of query_disc into ring
+ *                    tiles of W pixels (pix_tools.F90:712-800,
+ *                    pixel_routines.F90:1823-1879)
+ *  k_tile_paint      one workgroup per ring tile: every halo segment of the
+ *                    tile painted into an LDS tile (FP64, or FP32 two-sum),
+ *                    then one FP64 tile-to-map reduce per pixel
+ */
+#include "p2m_kernels.h"
+
+#include <hipcub/hipcub.hpp>
+
+/* ===================================================================== */
+/* helpers                                                                */
+/* ===================================================================== */
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0ULL;
+}
+__global__ void k_fill_f64(double *p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+__global__ void k_fill_f32(float *p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+static inline int fill_grid(int64_t n) { int64_t g = (n + 255) / 256; return (int)(g < 16384 ? (g < 1 ? 1 : g) : 16384); }
+
+void p2m_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v); }
+void p2m_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(256), 0, s, p, n); }
+void p2m_launch_fill_f64(double *p, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(fill_grid(n)), dim3(256), 0, s, p, n); }
+void p2m_launch_fill_f32(float *p, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3(fill_grid(n)), dim3(256), 0, s, p, n); }
+
+/* wave-aggregated append: returns the slot of this lane (or garbage if !take) */
+__device__ static inline unsigned long long wave_append(unsigned long long *counter, bool take)
+{
+    unsigned long long mask = __ballot(take);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (mask == 0) return 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+    return base + (unsigned long long)__popcll(below);
+}
+
+__device__ static inline void wave_count(unsigned long long *counter, unsigned long long v)
+{
+    /* sum v over the wave, one atomic */
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(counter, v);
+}
+
+/* ===================================================================== */
+/* decode                                                                 */
+/* ===================================================================== */
+
+__global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
+{
+    const CosmoDev &c = A.c;
+    const TabGeom &g = A.g;
+    unsigned long long n_cut = 0, n_rad = 0;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool painted = false;
+    HaloRec h;
+    if (i < A.n) {
+        const float *p = A.halos + 7 * i;
+        float p0 = p[0], p1 = p[1], p2 = p[2], p6 = p[6];
+        /* radial cut (pks2map.f90:144-160), painter axes x=pos3, y=pos2, z=pos1 */
+        double xh = (double)p2, yh = (double)p1, zh = (double)p0;
+        double xs = xh - A.chihview;
+        double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
+        if (!(chih > A.cut_high || chih < A.cut_low)) {
+            n_cut = 1;
+            double rth = (double)p6;
+            double mass = (((4.0 / 3.0) * c.mypi) * c.rho_0) * cube(rth);   /* pks2map.f90:264 */
+            double vh = 1.0;
+            if (A.profile == 2) {
+                double px = p0, py = p1, pz = p2;
+                double vr = (px * (double)p[3] + py * (double)p[4]) + pz * (double)p[5];
+                vr = vr / p2m_sqrt((px * px + py * py) + pz * pz);
+                vh = -vr / 3.0e5;
+            }
+            double chi_pre = p2m_sqrt((xh * xh + yh * yh) + zh * zh);          /* :289 */
+            double redshift = tab_lookup(c.zofr, c.nzofr, c.zofr_min, c.dzofr, chi_pre);  /* :290 */
+            double mh = p2m_sqrt(deltacrit_z(c, redshift) / A.dmodel) * mass;    /* :295 */
+            if (!(mass < A.mmin)) {                                            /* :296 */
+                double rh = p2m_pow(((((3.0 * mh) / 4.0) / c.mypi) / 200.0) / rhocrit_z(c, redshift), 1.0 / 3.0); /* :298 */
+                double dtheta = 4.0 * p2m_asin(rh / chih);                      /* haloproject.f90:47 */
+                if (!(dtheta >= 0.0 && dtheta <= HPX_PI)) {
+                    n_rad = 1;   /* pixel_routines.F90:1962 would abort */
+                } else {
+                    painted = true;
+                    double ux = xs / chih, uy = yh / chih, uz = zh / chih;
+                    h.hx = uz; h.hy = uy; h.hz = ux;
+                    h.dtheta = dtheta;
+                    h.vh = vh;
+                    h.cat = A.index_base + i;
+                    /* query_disc prologue (pixel_routines.F90:1981-2010) */
+                    double norm = p2m_sqrt((h.hx * h.hx + h.hy * h.hy) + h.hz * h.hz);
+                    double z0 = h.hz / norm;
+                    h.irmin = 0; h.irmax = -1;
+                    if (A.nside > 0) {
+                        double rlat0 = p2m_asin(z0);
+                        double rlat1 = rlat0 + dtheta, rlat2 = rlat0 - dtheta;
+                        double zmax = (rlat1 >= HPX_HALFPI) ? 1.0 : p2m_sin(rlat1);
+                        int32_t irmin = ring_num(A.nside, zmax);
+                        h.irmin = irmin - 1 > 1 ? irmin - 1 : 1;
+                        double zmin = (rlat2 <= -HPX_HALFPI) ? -1.0 : p2m_sin(rlat2);
+                        int32_t irmax = ring_num(A.nside, zmin);
+                        h.irmax = irmax + 1 < 4 * A.nside - 1 ? irmax + 1 : 4 * A.nside - 1;
+                    }
+                    /* discphirange_at_z prologue (pix_tools.F90:729-738) */
+                    h.cosang = p2m_cos(dtheta);
+                    double x0 = h.hx / norm, y0 = h.hy / norm;
+                    h.zz0 = h.hz / norm;
+                    h.phi0 = 0.0;
+                    if (x0 != 0.0 || y0 != 0.0) h.phi0 = p2m_atan2(y0, x0);
+                    h.a = x0 * x0 + y0 * y0;
+                    h.s0 = p2m_sqrt(h.a);
+                    double sh = p2m_sin(0.5 * dtheta);
+                    h.thr = 4.0 * (sh * sh);
+                    /* interpolate_table per-halo part (maptable.f90:184-202) */
+                    double chi = tab_lookup(c.rofz, c.nrofz, c.rofz_min, c.drofz, redshift);
+                    h.chi_r = chi;
+                    h.chi_out = chi > g.chimaxt ? 1 : 0;
+                    if (chi < g.chimint) chi = g.chimint + 1e-5;
+                    double mc = mh;
+                    if (mc < g.mhmint) mc = g.mhmint + 1e-5;
+                    if (mc > g.mhmaxt) mc = g.mhmaxt - 1e-5;
+                    double lc = p2m_log(chi), lm = p2m_log(mc);
+                    h.ichi = (int)((lc - g.lc0) / g.dchit) + 1;
+                    h.imh = (int)((lm - g.lm0) / g.dmht) + 1;
+                    h.fc = lc - (g.lc0 + (double)(h.ichi - 1) * g.dchit);
+                    h.fm = lm - (g.lm0 + (double)(h.imh - 1) * g.dmht);
+                    /* flat-sky bounding box (haloproject.f90:101-128) */
+                    h.flat = 0; h.th = 0.0; h.ph = 0.0;
+                    h.jmin = h.jmax = h.kmin = h.kmax = 0;
+                    if (A.npix_fs > 0 && !(uz < 0.0)) {
+                        double th = p2m_asin(ux), ph = p2m_asin(uy);
+                        h.th = th; h.ph = ph;
+                        double thetamin = th - 1.1 * dtheta, thetamax = th + 1.1 * dtheta;
+                        double phimin = ph - 1.1 * dtheta, phimax = ph + 1.1 * dtheta;
+                        double hf = A.fov / 2.0;
+                        if (!(thetamax < -hf || thetamin > hf) && !(phimax < -hf || phimin > hf)) {
+                            if (thetamin <= -hf) thetamin = -hf + 1e-10;
+                            if (thetamax >= hf) thetamax = hf - 1e-10;
+                            if (phimin <= -hf) phimin = -hf + 1e-10;
+                            if (phimax >= hf) phimax = hf - 1e-10;
+                            int jmin = (int)((thetamin + hf) / A.dpix) + 1; if (jmin < 1) jmin = 1;
+                            int jmax = (int)((thetamax + hf) / A.dpix) + 1; if (jmax > A.npix_fs) jmax = A.npix_fs;
+                            int kmin = (int)((phimin + hf) / A.dpix) + 1; if (kmin < 1) kmin = 1;
+                            int kmax = (int)((phimax + hf) / A.dpix) + 1; if (kmax > A.npix_fs) kmax = A.npix_fs;
+                            if (jmax >= jmin && kmax >= kmin) {
+                                h.flat = 1; h.jmin = jmin; h.jmax = jmax; h.kmin = kmin; h.kmax = kmax;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    unsigned long long slot = wave_append(A.n_out, painted);
+    if (painted && slot < (unsigned long long)A.cap) A.out[slot] = h;
+    wave_count(&A.counters[0], n_cut);
+    wave_count(&A.counters[2], n_rad);
+    wave_count(&A.counters[1], painted ? 1ULL : 0ULL);
+}
+
+void p2m_launch_decode(const DecodeArgs &a, hipStream_t s)
+{
+    if (a.n <= 0) return;
+    int64_t g = (a.n + 255) / 256;
+    hipLaunchKernelGGL(k_decode, dim3((unsigned)g), dim3(256), 0, s, a);
+}
+
+/* ===================================================================== */
+/* baseline: plain FP64 painter, one work item per halo                   */
+/* ===================================================================== */
+
+__global__ void __launch_bounds__(64) k_paint_plain_hp(PaintArgs A)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    unsigned long long n_list = 0, n_upd = 0, n_gaps = 0;
+    if (i < A.n_halo) {
+        HaloRec h = A.halos[i];
+        const int32_t nside = A.nside;
+        int last = 0, pend = 0;
+        for (int32_t ring = h.irmin; ring <= h.irmax; ++ring) {
+            int32_t npr, kshift; int64_t first;
+            ring_info(nside, ring, npr, kshift, first);
+            int32_t lo = 0;
+            int32_t np = disc_ring_run(h, nside, ring, npr, kshift, lo);
+            if (np == 0) { if (last) pend = 1; continue; }
+            if (pend) { n_gaps++; pend = 0; }
+            last = 1;
+            n_list += np;
+            for (int32_t k = 0; k < np; ++k) {
+                int32_t ip = lo + k; if (ip >= npr) ip -= npr;
+                int64_t pix = first + ip;
+                double v2[3];
+                pix2vec_ring(nside, pix, v2);
+                double angle = angdist(h.hx, h.hy, h.hz, v2[0], v2[1], v2[2]);
+                if (angle < h.dtheta) {
+                    n_upd++;
+                    for (int m = 0; m < A.n_maps; ++m) {
+                        double val = h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle);
+                        atomicAdd(&A.map64[m][pix], val);
+                    }
+                    if (A.count) {
+                        atomicAdd(&A.count[pix], 1u);
+                        atomicAdd(&A.hash[pix], (unsigned long long)halo_hash(h.cat));
+                    }
+                }
+            }
+        }
+    }
+    wave_count(&A.counters[0], n_list);
+    wave_count(&A.counters[1], n_upd);
+    wave_count(&A.counters[3], n_gaps);
+}
+
+void p2m_launch_paint_plain_hp(const PaintArgs &a, hipStream_t s)
+{
+    if (a.n_halo <= 0) return;
+    hipLaunchKernelGGL(k_paint_plain_hp, dim3((unsigned)((a.n_halo + 63) / 64)), dim3(64), 0, s, a);
+}
+
+/* ===================================================================== */
+/* flat-sky painter: one wave per halo, FP64, global atomics              */
+/* ===================================================================== */
+
+__global__ void __launch_bounds__(256) k_paint_fs(PaintArgs A, int plain)
+{
+    int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    int64_t hidx = blockIdx.x * 4LL + wave;
+    unsigned long long n_upd = 0;
+    if (plain) {
+        /* baseline: one work item per halo */
+        hidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+        if (hidx < A.n_halo) {
+            const HaloRec &hr = A.halos[hidx];
+            if (hr.flat) {
+                HaloRec h = hr;
+                double hf = A.fov / 2.0;
+                for (int k = h.kmin; k <= h.kmax; ++k) {
+                    double phip = -hf + ((double)k - 0.5) * A.dpix;
+                    for (int j = h.jmin; j <= h.jmax; ++j) {
+                        double thetap = -hf + ((double)j - 0.5) * A.dpix;
+                        double dphi = phip - h.ph, dth = thetap - h.th;
+                        double angle = p2m_sqrt(dphi * dphi + dth * dth);
+                        if (angle < h.dtheta) {
+                            int64_t pix = (int64_t)(j - 1) + (int64_t)A.npix_fs * (k - 1);
+                            n_upd++;
+                            for (int m = 0; m < A.n_maps; ++m)
+                                atomicAdd(&A.fs64[m][pix], h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle));
+                            if (A.fs_count) {
+                                atomicAdd(&A.fs_count[pix], 1u);
+                                atomicAdd(&A.fs_hash[pix], (unsigned long long)halo_hash(h.cat));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        wave_count(&A.counters[2], n_upd);
+        return;
+    }
+    if (hidx < A.n_halo && A.halos[hidx].flat) {
+        const HaloRec &h = A.halos[hidx];
+        int nj = h.jmax - h.jmin + 1, nk = h.kmax - h.kmin + 1;
+        int64_t tot = (int64_t)nj * nk;
+        double hf = A.fov / 2.0;
+        for (int64_t q = lane; q < tot; q += 64) {
+            int k = h.kmin + (int)(q / nj), j = h.jmin + (int)(q % nj);
+            double phip = -hf + ((double)k - 0.5) * A.dpix;
+            double thetap = -hf + ((double)j - 0.5) * A.dpix;
+            double dphi = phip - h.ph, dth = thetap - h.th;
+            double angle = p2m_sqrt(dphi * dphi + dth * dth);
+            if (angle < h.dtheta) {
+                int64_t pix = (int64_t)(j - 1) + (int64_t)A.npix_fs * (k - 1);
+                n_upd++;
+                for (int m = 0; m < A.n_maps; ++m)
+                    atomicAdd(&A.fs64[m][pix], h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle));
+                if (A.fs_count) {
+                    atomicAdd(&A.fs_count[pix], 1u);
+                    atomicAdd(&A.fs_hash[pix], (unsigned long long)halo_hash(h.cat));
+                }
+            }
+        }
+    }
+    wave_count(&A.counters[2], n_upd);
+}
+
+void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s)
+{
+    if (a.n_halo <= 0 || a.npix_fs <= 0) return;
+    if (plain) hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_halo + 255) / 256)), dim3(256), 0, s, a, 1);
+    else hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_halo + 3) / 4)), dim3(256), 0, s, a, 0);
+}
+
+/* ===================================================================== */
+/* binning of (halo, ring) runs into ring tiles                           */
+/* ===================================================================== */
+
+/* visit the tile pieces of every ring run of halo h:
+ *   f(ring, tile, lo_in_tile, len) */
+template <typename F>
+__device__ static inline void for_each_piece(const HaloRec &h, const BinArgs &B, F f, unsigned long long &n_list,
+                                             unsigned long long &n_gaps)
+{
+    int last = 0, pend = 0;
+    for (int32_t ring = h.irmin; ring <= h.irmax; ++ring) {
+        int32_t npr, kshift; int64_t first;
+        ring_info(B.nside, ring, npr, kshift, first);
+        int32_t lo = 0;
+        int32_t np = disc_ring_run(h, B.nside, ring, npr, kshift, lo);
+        if (np == 0) { if (last) pend = 1; continue; }
+        if (pend) { n_gaps++; pend = 0; }
+        last = 1;
+        n_list += np;
+        int32_t tb = B.tile_base[ring - 1];
+        int32_t a0 = lo, b0 = lo + np;           /* [a0, b0) possibly wrapping */
+        for (int piece = 0; piece < 2; ++piece) {
+            int32_t a, b;
+            if (piece == 0) { a = a0; b = b0 < npr ? b0 : npr; }
+            else { if (b0 <= npr) break; a = 0; b = b0 - npr; }
+            int32_t c0 = a >> B.log2w, c1 = (b - 1) >> B.log2w;
+            for (int32_t col = c0; col <= c1; ++col) {
+                int32_t s = col << B.log2w;
+                int32_t pa = a > s ? a : s;
+                int32_t pb = b < s + (1 << B.log2w) ? b : s + (1 << B.log2w);
+                f(ring, tb + col, pa - s, pb - pa);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    unsigned long long n_list = 0, n_gaps = 0, n_seg = 0;
+    if (i < B.n_halo) {
+        HaloRec h = B.halos[i];
+        for_each_piece(h, B, [&](int32_t, int32_t tile, int32_t, int32_t) {
+            atomicAdd(&B.tile_cnt[tile], 1u);
+            n_seg++;
+        }, n_list, n_gaps);
+    }
+    wave_count(&B.counters[0], n_list);
+    wave_count(&B.counters[3], n_gaps);
+    wave_count(&B.counters[4], n_seg);
+}
+
+__global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    unsigned long long n_list = 0, n_gaps = 0;
+    if (i < B.n_halo) {
+        HaloRec h = B.halos[i];
+        for_each_piece(h, B, [&](int32_t, int32_t tile, int32_t lo, int32_t len) {
+            uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
+            uint64_t pos = B.tile_off[tile] + slot;
+            B.segs[pos] = make_uint2((uint32_t)i, ((uint32_t)lo << 16) | (uint32_t)len);
+        }, n_list, n_gaps);
+    }
+}
+
+void p2m_launch_seg_count(const BinArgs &b, hipStream_t s)
+{
+    if (b.n_halo <= 0) return;
+    hipLaunchKernelGGL(k_seg_count, dim3((unsigned)((b.n_halo + 255) / 256)), dim3(256), 0, s, b);
+}
+void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s)
+{
+    if (b.n_halo <= 0) return;
+    hipLaunchKernelGGL(k_seg_emit, dim3((unsigned)((b.n_halo + 255) / 256)), dim3(256), 0, s, b);
+}
+
+/* exclusive scan of tile counts (uint32) into offsets (uint64) */
+int p2m_scan_tiles(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s)
+{
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, cnt, off, (int)n, s);
+    return e == hipSuccess ? 0 : -1;
+}
+
+/* ===================================================================== */
+/* tiled painter                                                          */
+/* ===================================================================== */
+
+#define SEG_BATCH 256
+
+/* per-segment constants staged in LDS */
+struct SegLDS {
+    uint32_t halo[SEG_BATCH];
+    uint32_t lo[SEG_BATCH];
+    uint32_t pre[SEG_BATCH + 1];
+};
+
+/* sin^2(x) for |x| <= pi/2 in FP64 (Taylor polynomials of p2m_math.h) */
+__device__ static inline double sin2_half(double x)
+{
+    double ax = p2m_fabs(x);
+    if (ax <= 0.7853981633974483) {
+        double s = p2m_sin_poly(x);
+        return s * s;
+    }
+    double y = (P2M_PIO2_HI - ax) + P2M_PIO2_LO;
+    double cc = p2m_cos_poly(y);
+    return cc * cc;
+}
+
+/* ln(1+q) for 0 <= q < 0.08 in FP32: 2 atanh(q/(2+q)) */
+__device__ static inline float log1p_small(float q)
+{
+    float s = q / (2.0f + q);
+    float s2 = s * s;
+    float p = __builtin_fmaf(s2, 1.0f / 7.0f, 1.0f / 5.0f);
+    p = __builtin_fmaf(s2, p, 1.0f / 3.0f);
+    return 2.0f * __builtin_fmaf(s * s2, p, s);
+}
+
+template <int MODE, int NM>
+__global__ void __launch_bounds__(256) k_tile_paint(TileArgs A)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int W = 1 << A.log2w;
+    /* LDS layout: acc (NM*W doubles, or NM*W*2 floats), count (W u32), hash (W u64), segs */
+    double *acc64 = (double *)smem_raw;
+    float *accS = (float *)smem_raw;
+    float *accC = accS + NM * W;
+    size_t acc_bytes = (size_t)NM * W * 8;
+    uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes);
+    unsigned long long *lhash = (unsigned long long *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 4 : 0));
+    size_t mem_bytes = A.count ? (size_t)W * 12 : 0;
+    SegLDS *sl = (SegLDS *)(smem_raw + acc_bytes + mem_bytes);
+
+    unsigned long long n_upd = 0, n_fb = 0;
+    for (int64_t t = blockIdx.x; t < A.n_tiles; t += gridDim.x) {
+        /* ring of this tile: largest r with tile_base[r] <= t */
+        int lo_r = 0, hi_r = A.n_rings - 1;
+        while (lo_r < hi_r) {
+            int mid = (lo_r + hi_r + 1) >> 1;
+            if (A.tile_base[mid] <= t) lo_r = mid; else hi_r = mid - 1;
+        }
+        const int32_t ring = lo_r + 1;
+        const int32_t col = (int32_t)(t - A.tile_base[lo_r]);
+        int32_t npr, kshift; int64_t first;
+        ring_info(A.nside, ring, npr, kshift, first);
+        const int32_t ip0 = col * W;
+        const int32_t width = npr - ip0 < W ? npr - ip0 : W;
+        double zr, sth;
+        ring_zsth(A.nside, ring, zr, sth);
+        double off, den;
+        ring_phi_param(A.nside, ring, off, den);
+
+        for (int q = threadIdx.x; q < width; q += blockDim.x) {
+            for (int m = 0; m < NM; ++m) {
+                if (MODE == 1) acc64[m * W + q] = 0.0;
+                else { accS[m * W + q] = 0.0f; accC[m * W + q] = 0.0f; }
+            }
+            if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
+        }
+
+        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
+        for (uint64_t b0 = sbeg; b0 < send;) {
+            __syncthreads();
+            int nb = (int)(send - b0 < SEG_BATCH ? send - b0 : SEG_BATCH);
+            /* stage the batch and scan its lengths (simple two-level scan) */
+            const int cap = (int)blockDim.x < SEG_BATCH ? (int)blockDim.x : SEG_BATCH;
+            if (nb > cap) nb = cap;
+            uint32_t len = 0;
+            if ((int)threadIdx.x < nb) {
+                uint2 sg = A.segs[b0 + threadIdx.x];
+                sl->halo[threadIdx.x] = sg.x;
+                sl->lo[threadIdx.x] = sg.y >> 16;
+                len = sg.y & 0xffffu;
+            }
+            /* inclusive wave scan */
+            uint32_t v = len;
+            int lane = __lane_id();
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t u = __shfl_up(v, o);
+                if (lane >= o) v += u;
+            }
+            __shared__ uint32_t wsum[16];
+            int wv = threadIdx.x >> 6;
+            if (lane == 63) wsum[wv] = v;
+            __syncthreads();
+            uint32_t add = 0;
+            for (int k = 0; k < wv; ++k) add += wsum[k];
+            if ((int)threadIdx.x < nb) sl->pre[threadIdx.x + 1] = v + add;
+            if (threadIdx.x == 0) sl->pre[0] = 0;
+            __syncthreads();
+            const uint32_t P = sl->pre[nb];
+            b0 += (uint64_t)nb;
+            for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+                /* segment of pixel p: largest s with pre[s] <= p */
+                int s_lo = 0, s_hi = nb - 1;
+                while (s_lo < s_hi) {
+                    int mid = (s_lo + s_hi + 1) >> 1;
+                    if (sl->pre[mid] <= p) s_lo = mid; else s_hi = mid - 1;
+                }
+                const int s = s_lo;
+                const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
+                const HaloRec &h = A.halos[sl->halo[s]];
+                const double ipd = (double)(ip0 + q);
+                const double phi = ((ipd + off) * HPX_HALFPI) / den;
+                bool member = false;
+                double vexact[NM];
+                float v32[NM];
+                bool use_exact = (MODE == 1);
+                if (MODE == 2) {
+                    double dphi = phi - h.phi0;
+                    if (dphi > HPX_PI) dphi -= HPX_TWOPI;
+                    double s2 = sin2_half(0.5 * dphi);
+                    double dz = zr - h.zz0, ds = sth - h.s0;
+                    double c2 = (dz * dz + ds * ds) + ((4.0 * sth) * h.s0) * s2;
+                    if (c2 >= h.thr * (1.0 + 1e-9)) {
+                        member = false;
+                    } else if (c2 > h.thr * (1.0 - 1e-9)) {
+                        use_exact = true;
+                    } else {
+                        member = true;
+                        double h2 = 0.25 * c2;
+                        double dd = 1.0 - 2.0 * h2;
+                        if (h.chi_out) {
+                            for (int m = 0; m < NM; ++m) v32[m] = 0.0f;
+                        } else if (!(dd > 0.0)) {
+                            use_exact = true;
+                        } else {
+                            double t2 = ((4.0 * h2) * (1.0 - h2)) / (dd * dd);
+                            double r2 = (h.chi_r * h.chi_r) * t2;
+                            float u = __builtin_amdgcn_logf((float)r2) * A.u_k1 - A.u_k0;
+                            float fl = __builtin_floorf(u);
+                            float fu = u - fl;
+                            if (u > A.u_max + A.band) {
+                                for (int m = 0; m < NM; ++m) v32[m] = 0.0f;   /* r > rmaxt */
+                            } else if ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
+                                       (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))) {
+                                use_exact = true;
+                            } else {
+                                int ir;
+                                float fr;
+                                if (u < 0.0f) {      /* r < rmint: r = rmint + 1e-5 (maptable.f90:190) */
+                                    ir = 1;
+                                    fr = A.fr_clamp;
+                                } else {
+                                    ir = (int)fl + 1;
+                                    double q2 = r2 * A.inv_node2[ir - 1] - 1.0;
+                                    fr = 0.5f * log1p_small((float)q2);
+                                }
+                                int64_t col0 = (int64_t)A.g.nrt * ((int64_t)(h.imh - 1) + (int64_t)A.g.nmht * (int64_t)(h.ichi - 1)) + (ir - 1);
+                                int64_t dmh = A.g.nrt, dch = (int64_t)A.g.nrt * A.g.nmht;
+                                float fc = (float)h.fc, fm = (float)h.fm;
+                                float w00 = (1.0f - fc) * (1.0f - fm), w10 = fc * (1.0f - fm);
+                                float w01 = (1.0f - fc) * fm, w11 = fc * fm;
+                                float vh = (float)h.vh;
+                                for (int m = 0; m < NM; ++m) {
+                                    const float *T = A.tab_prof[m];
+                                    float a0 = T[col0], a1 = T[col0 + 1];
+                                    float b0v = T[col0 + dmh], b1 = T[col0 + dmh + 1];
+                                    float c0v = T[col0 + dch], c1 = T[col0 + dch + 1];
+                                    float d0 = T[col0 + dmh + dch], d1 = T[col0 + dmh + dch + 1];
+                                    float ea = __builtin_fmaf(fr, a1 - a0, a0);
+                                    float eb = __builtin_fmaf(fr, b1 - b0v, b0v);
+                                    float ec = __builtin_fmaf(fr, c1 - c0v, c0v);
+                                    float ed = __builtin_fmaf(fr, d1 - d0, d0);
+                                    float val = w00 * ea + w10 * eb + w01 * ec + w11 * ed;
+                                    v32[m] = vh * val;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (use_exact) {
+                    double s, cph;
+                    p2m_sincos(phi, &s, &cph);
+                    double angle = angdist(h.hx, h.hy, h.hz, sth * cph, sth * s, zr);
+                    member = angle < h.dtheta;
+                    if (member) {
+                        for (int m = 0; m < NM; ++m)
+                            vexact[m] = h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle);
+                        if (MODE == 2) {
+                            n_fb++;
+                            for (int m = 0; m < NM; ++m) v32[m] = (float)vexact[m];
+                        }
+                    }
+                }
+                if (member) {
+                    n_upd++;
+                    for (int m = 0; m < NM; ++m) {
+                        if (MODE == 1) {
+                            atomicAdd(&acc64[m * W + q], vexact[m]);
+                        } else {
+                            float vv = v32[m];
+                            float old = atomicAdd(&accS[m * W + q], vv);
+                            float sum = old + vv;
+                            float bb = sum - old;
+                            float err = (old - (sum - bb)) + (vv - bb);
+                            atomicAdd(&accC[m * W + q], err);
+                        }
+                    }
+                    if (A.count) {
+                        atomicAdd(&lcount[q], 1u);
+                        atomicAdd(&lhash[q], (unsigned long long)halo_hash(h.cat));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        /* tile-to-map reduce */
+        const int64_t pix0 = first + ip0;
+        for (int q = threadIdx.x; q < width; q += blockDim.x) {
+            for (int m = 0; m < NM; ++m) {
+                if (MODE == 1) {
+                    double v = acc64[m * W + q];
+                    double *dst = A.map64[m] + pix0 + q;
+                    *dst = A.accumulate ? *dst + v : v;
+                } else {
+                    double v = (double)accS[m * W + q] + (double)accC[m * W + q];
+                    float *dst = A.map32[m] + pix0 + q;
+                    *dst = A.accumulate ? (float)((double)*dst + v) : (float)v;
+                }
+            }
+            if (A.count) {
+                uint32_t *dc = A.count + pix0 + q;
+                unsigned long long *dh = A.hash + pix0 + q;
+                *dc = A.accumulate ? *dc + lcount[q] : lcount[q];
+                *dh = A.accumulate ? *dh + lhash[q] : lhash[q];
+            }
+        }
+        __syncthreads();
+    }
+    wave_count(&A.counters[1], n_upd);
+    wave_count(&A.counters[5], n_fb);
+}
+
+size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership)
+{
+    size_t W = (size_t)1 << log2w;
+    size_t b = (size_t)n_maps * W * 8;
+    if (membership) b += W * 12;
+    b += sizeof(SegLDS);
+    return b;
+}
+
+template <int MODE, int NM>
+static void launch_tile(const TileArgs &a, int grid, int wg, size_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_tile_paint<MODE, NM>), dim3(grid), dim3(wg), lds, s, a);
+}
+
+int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s)
+{
+    size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr);
+    if (lds > 160 * 1024) return -1;
+#define P2M_TL(MD, NMv) case NMv: launch_tile<MD, NMv>(a, grid, wg, lds, s); break;
+    if (mode == 1) {
+        switch (n_maps) { P2M_TL(1, 1) P2M_TL(1, 2) P2M_TL(1, 3) P2M_TL(1, 4) P2M_TL(1, 5) P2M_TL(1, 6) default: return -1; }
+    } else {
+        switch (n_maps) { P2M_TL(2, 1) P2M_TL(2, 2) P2M_TL(2, 3) P2M_TL(2, 4) P2M_TL(2, 5) P2M_TL(2, 6) default: return -1; }
+    }
+#undef P2M_TL
+    return 0;
+}
+
+/* ===================================================================== */
+/* products: moments, RMS, dP/dy (pks2map.f90:321-357)                    */
+/* ===================================================================== */
+
+__global__ void __launch_bounds__(256) k_moments(const float *m32, const double *m64, int64_t n, double mean,
+                                                 double *out /* [0..2]=sum x^k, [3]=sum (x-mean)^2 */,
+                                                 unsigned long long *hist, int nb, float dy)
+{
+    double s1 = 0, s2 = 0, s3 = 0, sd = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float xf = m32 ? m32[i] : (float)m64[i];
+        double x = (double)xf;
+        s1 += x; s2 += x * x; s3 += (x * x) * x;
+        double d = x - mean; sd += d * d;
+        if (hist) {
+            /* m = int(hpmap(i)/dpdy_dy)+1 in FP32 (pks2map.f90:344) */
+            float u = xf / dy;
+            if (u > -1.0f && u < (float)nb) {
+                int mb = (int)u + 1;
+                if (mb > 0 && mb <= nb) atomicAdd(&hist[mb - 1], 1ULL);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o); s2 += __shfl_xor(s2, o); s3 += __shfl_xor(s3, o); sd += __shfl_xor(sd, o);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&out[0], s1); atomicAdd(&out[1], s2); atomicAdd(&out[2], s3); atomicAdd(&out[3], sd);
+    }
+}
+
+void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double mean, double *out,
+                        unsigned long long *hist, int nb, float dy, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_moments, dim3(fill_grid(n) < 2048 ? fill_grid(n) : 2048), dim3(256), 0, s, m32, m64, n, mean, out, hist, nb, dy);
+}
+
+__global__ void k_f64_to_f32(const double *in, float *out, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+__global__ void k_f32_to_f64(const float *in, double *out, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = (double)in[i];
+}
+void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_f64_to_f32, dim3(fill_grid(n)), dim3(256), 0, s, in, out, n); }
+void p2m_launch_f32_to_f64(const float *in, double *out, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_f32_to_f64, dim3(fill_grid(n)), dim3(256), 0, s, in, out, n); }
