@@ -60,11 +60,16 @@ typedef struct {
     int32_t flags;        /* P2M_FLAG_* */
 } p2m_config;
 
+/* run flags */
+#define P2M_RUN_PRECUT       1  /* halos were already radially cut by the caller (pks2map.f90:144-160),
+                                   e.g. before the scramble/centre branches; decode skips the cut */
+
 typedef struct {
     double omegam, omegal, h;   /* pks2map.f90:118-121 */
     double zmin, zmax;          /* radial cut (pks2map.f90:73,79,134-135) */
     double chihview;            /* observer shift along painter x (pks2map.f90:80,149,292) */
     double mmin;                /* mass cut (pks2map.f90:84) */
+    int32_t flags;              /* P2M_RUN_* */
 } p2m_run;
 
 typedef struct {
@@ -119,7 +124,7 @@ int p2m_fetch_membership(p2m_ctx *ctx, uint32_t *hp_count, uint64_t *hp_hash,
 int p2m_get_stats(p2m_ctx *ctx, p2m_stats *st);
 
 /* Tile shape of the tiled painter: width in pixels along a ring (power of 2,
- * 64..4096) and workgroup size (64..1024).  0,0 = autotune on first paint. */
+ * 64..4096) and workgroup size (64..512).  0,0 = autotune on first paint. */
 int p2m_set_tile(p2m_ctx *ctx, int32_t tile_w, int32_t tile_wg);
 
 /* Products of pks2map.f90:321-357 computed on the device from map slot
@@ -137,6 +142,12 @@ int p2m_write_outputs(p2m_ctx *ctx, int slot, const char *base, int write_fs, in
 int p2m_make_maptable(int32_t device, int32_t model, double nu, double alpha_cmb,
                       int32_t nchit, int32_t nmht, int32_t nrt,
                       int32_t hdr_i[3], float hdr_f[6], float *table, int64_t n_table);
+
+/* Diagnostic (no counterpart in the reference): evaluate the library's FP64
+ * elementary function `op` on the device for n inputs, so callers can check
+ * that device and host produce the same bits.  op: 0 sqrt, 1 sin, 2 cos,
+ * 3 tan, 4 asin, 5 acos, 6 atan2(x, y), 7 log, 8 exp, 9 pow(x, y). */
+int p2m_selftest_math(int32_t device, int32_t op, const double *x, const double *y, double *out, int64_t n);
 
 /* Library build identification. */
 const char *p2m_version(void);

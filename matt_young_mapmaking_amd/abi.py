@@ -16,13 +16,15 @@ PREC_PLAIN_FP64 = 0
 PREC_FP64 = 1
 PREC_FP32_COMP = 2
 FLAG_MEMBERSHIP = 1
+RUN_PRECUT = 1
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "size mismatch", -3: "HIP error", -4: "allocation failed",
           -5: "radius", -6: "state", -7: "io", -8: "no GPU"}
 
 EXPORTS = ["p2m_create", "p2m_destroy", "p2m_strerror", "p2m_set_run", "p2m_rofz", "p2m_zofr", "p2m_load_table",
            "p2m_paint", "p2m_paint_device", "p2m_clear", "p2m_fetch_maps", "p2m_fetch_membership", "p2m_get_stats",
-           "p2m_set_tile", "p2m_products", "p2m_write_outputs", "p2m_make_maptable", "p2m_version"]
+           "p2m_set_tile", "p2m_products", "p2m_write_outputs", "p2m_make_maptable", "p2m_selftest_math",
+           "p2m_version"]
 
 
 class Config(C.Structure):
@@ -33,7 +35,7 @@ class Config(C.Structure):
 
 class Run(C.Structure):
     _fields_ = [("omegam", C.c_double), ("omegal", C.c_double), ("h", C.c_double), ("zmin", C.c_double),
-                ("zmax", C.c_double), ("chihview", C.c_double), ("mmin", C.c_double)]
+                ("zmax", C.c_double), ("chihview", C.c_double), ("mmin", C.c_double), ("flags", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -82,6 +84,7 @@ def load():
     L.p2m_write_outputs.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int]
     L.p2m_make_maptable.argtypes = [i32, i32, d, d, i32, i32, i32, P(i32), P(C.c_float), P(C.c_float), i64]
     L.p2m_version.restype = C.c_char_p
+    L.p2m_selftest_math.argtypes = [i32, i32, P(d), P(d), P(d), i64]
     _lib = L
     return L
 
@@ -129,8 +132,8 @@ class Painter:
     def __exit__(self, *a):
         self.close()
 
-    def set_run(self, zmin=0.05, zmax=1.2, chihview=0.0, mmin=1e13, omegam=0.25, omegal=0.75, h=0.7):
-        r = Run(omegam, omegal, h, zmin, zmax, chihview, mmin)
+    def set_run(self, zmin=0.05, zmax=1.2, chihview=0.0, mmin=1e13, omegam=0.25, omegal=0.75, h=0.7, precut=False):
+        r = Run(omegam, omegal, h, zmin, zmax, chihview, mmin, RUN_PRECUT if precut else 0)
         _chk(self.L.p2m_set_run(self.ctx, C.byref(r)), "p2m_set_run")
 
     def rofz(self, z):
@@ -210,3 +213,17 @@ def make_maptable(model=1, nu=148.0, alpha=0.0, nchit=100, nmht=100, nrt=200, de
     _chk(L.p2m_make_maptable(device, model, nu, alpha, nchit, nmht, nrt, _p(hi, C.c_int32), _p(hf, C.c_float),
                              _p(t, C.c_float), t.size), "p2m_make_maptable")
     return hi, hf, t
+
+
+MATH_OPS = ["sqrt", "sin", "cos", "tan", "asin", "acos", "atan2", "log", "exp", "pow"]
+
+
+def selftest_math(op, x, y=None, device=0):
+    """Evaluate the library's portable FP64 function `op` on the device."""
+    L = load()
+    x = np.ascontiguousarray(x, np.float64)
+    yy = None if y is None else np.ascontiguousarray(y, np.float64)
+    out = np.zeros_like(x)
+    _chk(L.p2m_selftest_math(device, MATH_OPS.index(op), _p(x, C.c_double), _p(yy, C.c_double),
+                             _p(out, C.c_double), x.size), "p2m_selftest_math")
+    return out

@@ -21,10 +21,21 @@
 #include "../../include/pks2map_gpu.h"
 #include "p2m_kernels.h"
 
-#define HIPCHK(x)                                   \
-    do {                                            \
-        hipError_t e_ = (x);                        \
-        if (e_ != hipSuccess) return P2M_E_HIP;     \
+/* last HIP failure of this thread, reported by p2m_strerror(P2M_E_HIP) */
+static thread_local char g_hip_detail[256] = "";
+
+static int hip_fail(hipError_t e, const char *what, int line)
+{
+    snprintf(g_hip_detail, sizeof(g_hip_detail), "HIP runtime error: %s (%s) at p2m_api.hip:%d: %s",
+             hipGetErrorName(e), hipGetErrorString(e), line, what);
+    (void)hipGetLastError();
+    return P2M_E_HIP;
+}
+
+#define HIPCHK(x)                                                  \
+    do {                                                           \
+        hipError_t e_ = (x);                                       \
+        if (e_ != hipSuccess) return hip_fail(e_, #x, __LINE__);   \
     } while (0)
 
 namespace {
@@ -62,7 +73,8 @@ struct p2m_ctx {
     double *d_fs64[P2M_MAXMAPS] = {nullptr};
     uint32_t *d_count = nullptr, *d_fs_count = nullptr;
     unsigned long long *d_hash = nullptr, *d_fs_hash = nullptr;
-    bool hp_zero = true;
+    bool hp_zero = true;        /* next paint may store instead of accumulate */
+    bool hp_valid = true;       /* device HEALPix arrays hold the map (false after a lazy clear) */
     HaloRec *d_halo = nullptr;
     int64_t halo_cap = 0;
     unsigned long long *d_ctr = nullptr;
@@ -156,7 +168,7 @@ const char *p2m_strerror(int code)
     case P2M_OK: return "ok";
     case P2M_E_INVALID: return "invalid argument";
     case P2M_E_SIZE: return "buffer or header size mismatch";
-    case P2M_E_HIP: return "HIP runtime error";
+    case P2M_E_HIP: return g_hip_detail[0] ? g_hip_detail : "HIP runtime error";
     case P2M_E_NOMEM: return "allocation failed";
     case P2M_E_RADIUS: return "disc radius outside [0, pi]";
     case P2M_E_STATE: return "call out of order (run parameters or table missing)";
@@ -235,18 +247,37 @@ int p2m_clear(p2m_ctx *c)
 {
     if (!c) return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
+    /* the tiled painters store every HEALPix pixel on the first paint after a
+     * clear, so only the plain painter (atomics) needs a zeroed map */
+    const bool tiled = c->cfg.precision != P2M_PREC_PLAIN_FP64;
     for (int m = 0; m < c->cfg.n_maps; ++m) {
-        if (c->d_hp64[m]) p2m_launch_fill_f64(c->d_hp64[m], c->hpnpix, c->stream);
-        if (c->d_hp32[m]) p2m_launch_fill_f32(c->d_hp32[m], c->hpnpix, c->stream);
+        if (c->d_hp64[m] && !tiled) p2m_launch_fill_f64(c->d_hp64[m], c->hpnpix, c->stream);
         if (c->d_fs64[m]) p2m_launch_fill_f64(c->d_fs64[m], c->fsnpix, c->stream);
     }
-    if (c->d_count) p2m_launch_fill_u32(c->d_count, c->hpnpix, 0u, c->stream);
-    if (c->d_hash) p2m_launch_fill_u64(c->d_hash, c->hpnpix, c->stream);
+    if (c->d_count && !tiled) p2m_launch_fill_u32(c->d_count, c->hpnpix, 0u, c->stream);
+    if (c->d_hash && !tiled) p2m_launch_fill_u64(c->d_hash, c->hpnpix, c->stream);
     if (c->d_fs_count) p2m_launch_fill_u32(c->d_fs_count, c->fsnpix, 0u, c->stream);
     if (c->d_fs_hash) p2m_launch_fill_u64(c->d_fs_hash, c->fsnpix, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->hp_zero = true;
+    c->hp_valid = !tiled;
+    return 0;
+}
+
+/* Zero-fill the HEALPix arrays if a lazy clear left them unwritten. */
+static int ensure_hp_valid(p2m_ctx *c)
+{
+    if (c->hp_valid) return 0;
+    for (int m = 0; m < c->cfg.n_maps; ++m) {
+        if (c->d_hp64[m]) p2m_launch_fill_f64(c->d_hp64[m], c->hpnpix, c->stream);
+        if (c->d_hp32[m]) p2m_launch_fill_f32(c->d_hp32[m], c->hpnpix, c->stream);
+    }
+    if (c->d_count) p2m_launch_fill_u32(c->d_count, c->hpnpix, 0u, c->stream);
+    if (c->d_hash) p2m_launch_fill_u64(c->d_hash, c->hpnpix, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->hp_valid = true;
     return 0;
 }
 
@@ -360,7 +391,7 @@ int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
     if (!c) return P2M_E_INVALID;
     if (tile_w == 0 && tile_wg == 0) { c->tuned = false; return 0; }
     if (!is_pow2(tile_w) || tile_w < 64 || tile_w > 4096) return P2M_E_INVALID;
-    if (tile_wg < 64 || tile_wg > 1024 || tile_wg % 64) return P2M_E_INVALID;
+    if (tile_wg < 64 || tile_wg > 512 || tile_wg % 64) return P2M_E_INVALID;
     int l = 0;
     while ((1 << l) < tile_w) ++l;
     if (p2m_tile_lds_bytes(c->cfg.precision == P2M_PREC_FP64 ? 1 : 2, c->cfg.n_maps, l, c->d_count != nullptr) > 160 * 1024)
@@ -392,6 +423,7 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
     a.halos = d_src; a.n = n; a.index_base = base;
     a.c = c->cdev; a.g = c->g;
     a.cut_low = c->cut_low; a.cut_high = c->cut_high; a.chihview = c->run.chihview; a.mmin = c->run.mmin;
+    a.precut = (c->run.flags & P2M_RUN_PRECUT) ? 1 : 0;
     a.dmodel = c->cfg.model == 2 ? 500.0 : 200.0;
     a.profile = c->cfg.profile; a.nside = c->cfg.nside; a.npix_fs = c->cfg.npix_fs;
     a.fov = ((c->cfg.fov_deg / 360.0) * 2.0) * c->mypi;        /* pks2map.f90:92 */
@@ -548,6 +580,7 @@ static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n,
     int64_t n_halo = (int64_t)std::min<unsigned long long>(n_out, (unsigned long long)c->halo_cap);
     rc = paint_halos(c, n_halo, allow_tune);
     if (rc) return rc;
+    c->hp_valid = true;
     c->st.n_in = n;
     return finish_stats(c);
 }
@@ -576,7 +609,7 @@ int p2m_fetch_maps(p2m_ctx *c, int slot, float *hp32, double *hp64, float *fs32,
     if (!c) return P2M_E_INVALID;
     if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    { int rc = ensure_hp_valid(c); if (rc) return rc; }
     if (c->hpnpix > 0) {
         if (c->d_hp64[slot]) {
             if (hp64) HIPCHK(hipMemcpy(hp64, c->d_hp64[slot], sizeof(double) * c->hpnpix, hipMemcpyDeviceToHost));
@@ -608,7 +641,7 @@ int p2m_fetch_membership(p2m_ctx *c, uint32_t *hc, uint64_t *hh, uint32_t *fc, u
     if (!c) return P2M_E_INVALID;
     if (!(c->cfg.flags & P2M_FLAG_MEMBERSHIP)) return P2M_E_STATE;
     HIPCHK(hipSetDevice(c->cfg.device));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    { int rc = ensure_hp_valid(c); if (rc) return rc; }
     if (c->hpnpix > 0) {
         if (hc) HIPCHK(hipMemcpy(hc, c->d_count, sizeof(uint32_t) * c->hpnpix, hipMemcpyDeviceToHost));
         if (hh) HIPCHK(hipMemcpy(hh, c->d_hash, sizeof(uint64_t) * c->hpnpix, hipMemcpyDeviceToHost));
@@ -651,7 +684,9 @@ static int autotune(p2m_ctx *c, int64_t n_halo)
         int rc = build_tiles(c, cd[0]); if (rc) return rc;
         c->wg = cd[1]; c->tuned = true;
         c->hp_zero = true;
-        rc = paint_halos(c, n_halo, false); if (rc) return rc;
+        rc = paint_halos(c, n_halo, false);
+        if (rc == P2M_E_INVALID) { (void)p2m_clear(c); continue; }
+        if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->stream));
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[4]);
@@ -673,6 +708,7 @@ int p2m_products(p2m_ctx *c, int slot, double moments[6], double rms[2], float d
     if (!c) return P2M_E_INVALID;
     if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
+    { int rc = ensure_hp_valid(c); if (rc) return rc; }
     double *d_out = nullptr;
     unsigned long long *d_hist = nullptr;
     if (dmalloc((void **)&d_out, sizeof(double) * 8)) return P2M_E_NOMEM;
@@ -853,4 +889,28 @@ extern "C" int p2m_write_outputs(p2m_ctx *c, int slot, const char *base, int wri
         if (rc) return rc;
     }
     return 0;
+}
+
+extern "C" int p2m_selftest_math(int32_t device, int32_t op, const double *x, const double *y, double *out, int64_t n)
+{
+    if (!x || !out || n < 0 || op < 0 || op > 9) return P2M_E_INVALID;
+    if ((op == 6 || op == 9) && !y) return P2M_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { (void)hipGetLastError(); return P2M_E_NODEVICE; }
+    if (device < 0 || device >= ndev) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(device));
+    if (n == 0) return 0;
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    size_t b = sizeof(double) * (size_t)n;
+    int rc = dmalloc((void **)&dx, b);
+    if (!rc && y) rc = dmalloc((void **)&dy, b);
+    if (!rc) rc = dmalloc((void **)&dout, b);
+    if (!rc && hipMemcpy(dx, x, b, hipMemcpyHostToDevice) != hipSuccess) rc = P2M_E_HIP;
+    if (!rc && y && hipMemcpy(dy, y, b, hipMemcpyHostToDevice) != hipSuccess) rc = P2M_E_HIP;
+    if (!rc && p2m_launch_selftest_math(op, dx, dy, dout, n, 0)) rc = P2M_E_HIP;
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = P2M_E_HIP;
+    if (!rc && hipMemcpy(out, dout, b, hipMemcpyDeviceToHost) != hipSuccess) rc = P2M_E_HIP;
+    dfree(dx); dfree(dy); dfree(dout);
+    (void)hipGetLastError();
+    return rc;
 }

@@ -15,6 +15,7 @@ struct DecodeArgs {
     CosmoDev c;
     TabGeom g;
     double cut_low, cut_high, chihview, mmin, dmodel;
+    int32_t precut;
     int32_t profile, nside, npix_fs;
     double fov, dpix;
     HaloRec *out;
@@ -88,6 +89,8 @@ void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double m
                         unsigned long long *hist, int nb, float dy, hipStream_t s);
 void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s);
 void p2m_launch_f32_to_f64(const float *in, double *out, int64_t n, hipStream_t s);
+
+int p2m_launch_selftest_math(int op, const double *x, const double *y, double *out, int64_t n, hipStream_t s);
 
 /* make_maptable kernel (p2m_maptable.hip) */
 int p2m_launch_make_maptable(int32_t model, double nu, double alpha_cmb, int32_t nchit, int32_t nmht, int32_t nrt,

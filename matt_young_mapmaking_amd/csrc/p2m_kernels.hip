@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
         double xh = (double)p2, yh = (double)p1, zh = (double)p0;
         double xs = xh - A.chihview;
         double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
-        if (!(chih > A.cut_high || chih < A.cut_low)) {
+        if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
             n_cut = 1;
             double rth = (double)p6;
             double mass = (((4.0 / 3.0) * c.mypi) * c.rho_0) * cube(rth);   /* pks2map.f90:264 */
@@ -438,7 +438,7 @@ __device__ static inline float log1p_small(float q)
 }
 
 template <int MODE, int NM>
-__global__ void __launch_bounds__(256) k_tile_paint(TileArgs A)
+__global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
 {
     extern __shared__ unsigned char smem_raw[];
     const int W = 1 << A.log2w;
@@ -729,3 +729,35 @@ __global__ void k_f32_to_f64(const float *in, double *out, int64_t n)
 }
 void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_f64_to_f32, dim3(fill_grid(n)), dim3(256), 0, s, in, out, n); }
 void p2m_launch_f32_to_f64(const float *in, double *out, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_f32_to_f64, dim3(fill_grid(n)), dim3(256), 0, s, in, out, n); }
+
+/* ===================================================================== */
+/* diagnostic: portable math on the device                                */
+/* ===================================================================== */
+
+__global__ void k_selftest_math(int op, const double *x, const double *y, double *out, int64_t n)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = x[i], b = y ? y[i] : 0.0, r = 0.0;
+    switch (op) {
+    case 0: r = p2m_sqrt(a); break;
+    case 1: r = p2m_sin(a); break;
+    case 2: r = p2m_cos(a); break;
+    case 3: r = p2m_tan(a); break;
+    case 4: r = p2m_asin(a); break;
+    case 5: r = p2m_acos(a); break;
+    case 6: r = p2m_atan2(a, b); break;
+    case 7: r = p2m_log(a); break;
+    case 8: r = p2m_exp(a); break;
+    case 9: r = p2m_pow(a, b); break;
+    default: r = 0.0; break;
+    }
+    out[i] = r;
+}
+
+int p2m_launch_selftest_math(int op, const double *x, const double *y, double *out, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_selftest_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, op, x, y, out, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

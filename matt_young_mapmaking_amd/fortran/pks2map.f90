@@ -1,0 +1,341 @@
+! pks2map.f90 -- drop-in pks2map driver over the MI355X painter (libp2m.so).
+!
+! Same command line, stdout lines and output files as the reference program
+! src/pks2map_src/pks2map.f90:
+!   pks2map <filein> <fileout> <tablefile> [<zmin> <nside> <scramble> <center>
+!           <npix> <fov> <zmax> <chihview> <model> <profile>]
+! The host keeps the CLI (:62-82), the hard-coded cosmology (:118-128), the
+! catalogue read (pksc.f90:22-90 for one rank), the table read
+! (maptable.f90:150-172), the optional scramble/centre branches (:162-262)
+! and the reporting; the halo loop (:276-303), the products (:321-357) and
+! the writers (:359-381) run through the C ABI.
+!
+! Extensions (environment, so the CLI stays the reference's):
+!   P2M_PRECISION = fp32 (default) | fp64 | plain
+!   P2M_TILE      = "W,WG" tile shape of the tiled painter (default: autotuned)
+! <tablefile> may be a comma-separated list: all tables are painted in one
+! catalogue pass; map k is written as <fileout>_<table basename>_{fs.map,hp.fits,dp.bin}.
+! Inputs are validated before the GPU is touched; any failure exits with a
+! nonzero status and writes no output file.
+program pks2map
+  use iso_c_binding
+  use p2m_iface
+  use p2m_args
+  implicit none
+
+  character(len=1024) :: filein, fileout, tablearg, base, env
+  character(len=1024), allocatable :: tables(:)
+  integer :: ntab, k, ios, u, i, m, p0, p1
+  real :: zmin, fov, zmax, chihview, mypi, dpix
+  integer :: hpnside, scramble, centre, npix, model, profile
+  integer(c_int) :: rc
+  type(p2m_config) :: cfg
+  type(p2m_run) :: run
+  type(p2m_stats) :: st
+  type(c_ptr) :: ctx
+  integer(c_int32_t) :: nhalo_hdr
+  integer(c_int32_t) :: hdr_i(3), hdr_i0(3)
+  real(c_float) :: rthmax, hdr_f(6), hdr_f0(6)
+  real(c_float), allocatable :: halos(:,:)
+  real(c_float), allocatable :: tabs(:,:)
+  integer(c_int64_t) :: nh, ntv, fsize
+  real(c_double) :: moments(6), rms(2), chi_lo, chi_hi
+  real(c_float) :: dpdy(10000)
+  integer(c_int64_t) :: hpnpix
+
+  if (command_argument_count() < 3) then
+     write(*, '(a)') 'Usage: pks2map <filein> <fileout> <tablefile> [<zmin> <nside> <scramble> <center> ' // &
+          '<npix> <fov> <zmax> <chihview> <model> <profile>]'
+     stop
+  end if
+
+  call get_command_argument(1, filein)
+  call get_command_argument(2, fileout)
+  call get_command_argument(3, tablearg)
+  zmin     = r4arg(4, 0.0)
+  hpnside  = i4arg(5, 4096)
+  scramble = i4arg(6, 0)
+  centre   = i4arg(7, 0)
+  npix     = i4arg(8, 16)
+  fov      = r4arg(9, 10.0)
+  zmax     = r4arg(10, 1.25)
+  chihview = r4arg(11, 0.0)
+  model    = i4arg(12, 1)
+  profile  = i4arg(13, 1)
+
+  mypi = 4. * atan(1.0)
+  fov  = fov / 360 * 2 * mypi
+  dpix = fov / npix
+  write(*, 12) npix, npix, dpix, fov / 2 / mypi * 360.
+12 format(/, /, 'Resolution:         ', i0, ' x ', i0, ' pixels (', 1pe8.2, ' radians)', /, &
+          'Field of view:      ', 1pe8.2, ' degrees')
+  hpnpix = -1
+  if (hpnside >= 1 .and. iand(hpnside - 1, hpnside) == 0) hpnpix = 12_8 * hpnside * hpnside
+  write(*, 13) hpnside, hpnpix
+13 format('Healpix Nside:      ', i0, /, 'Number of pixels:   ', i0)
+  if (hpnpix < 0) call fail('nside must be a power of 2', 2)
+
+  ! ---------------------------------------------------------------- catalogue
+  open(newunit=u, file=trim(filein), access='stream', form='unformatted', status='old', action='read', iostat=ios)
+  if (ios /= 0) call fail('cannot open catalogue '//trim(filein), 3)
+  inquire(unit=u, size=fsize)
+  read(u, iostat=ios) nhalo_hdr, rthmax
+  if (ios /= 0 .or. nhalo_hdr < 0) call fail('bad catalogue header in '//trim(filein), 3)
+  if (fsize /= 8_8 + 28_8 * int(nhalo_hdr, 8)) call fail('catalogue size does not match its header (truncated?)', 3)
+  nh = nhalo_hdr
+  allocate(halos(7, max(nh, 1_8)))
+  if (nh > 0) then
+     read(u, pos=9, iostat=ios) halos(:, 1:nh)
+     if (ios /= 0) call fail('catalogue read failed', 3)
+  end if
+  close(u)
+
+  ! ---------------------------------------------------------------- tables
+  ntab = 1
+  do i = 1, len_trim(tablearg)
+     if (tablearg(i:i) == ',') ntab = ntab + 1
+  end do
+  if (ntab > 6) call fail('at most 6 tables per pass', 2)
+  allocate(tables(ntab))
+  p0 = 1
+  k = 0
+  do i = 1, len_trim(tablearg) + 1
+     if (i > len_trim(tablearg) .or. tablearg(min(i, len(tablearg)):min(i, len(tablearg))) == ',') then
+        k = k + 1
+        tables(k) = tablearg(p0:i - 1)
+        p0 = i + 1
+     end if
+  end do
+  do k = 1, ntab
+     open(newunit=u, file=trim(tables(k)), access='stream', form='unformatted', status='old', action='read', iostat=ios)
+     if (ios /= 0) call fail('cannot open table '//trim(tables(k)), 3)
+     inquire(unit=u, size=fsize)
+     read(u, iostat=ios) hdr_i, hdr_f
+     if (ios /= 0 .or. any(hdr_i < 2)) call fail('bad table header in '//trim(tables(k)), 3)
+     ntv = 2_8 * hdr_i(1) * hdr_i(2) * hdr_i(3)
+     if (fsize /= 36_8 + 4_8 * ntv) call fail('table size does not match its header: '//trim(tables(k)), 3)
+     if (k == 1) then
+        hdr_i0 = hdr_i
+        hdr_f0 = hdr_f
+        allocate(tabs(ntv, ntab))
+     else if (any(hdr_i /= hdr_i0) .or. any(hdr_f /= hdr_f0)) then
+        call fail('all tables of one pass must share the grid', 3)
+     end if
+     read(u, pos=37, iostat=ios) tabs(:, k)
+     if (ios /= 0) call fail('table read failed', 3)
+     close(u)
+  end do
+
+  ! ---------------------------------------------------------------- GPU context
+  cfg%nside = hpnside
+  cfg%npix_fs = npix
+  cfg%fov_deg = dble(fov / 2 / mypi * 360.)
+  cfg%fov_deg = dble(r4arg(9, 10.0))
+  cfg%model = model
+  cfg%profile = profile
+  cfg%n_maps = ntab
+  cfg%precision = P2M_PREC_FP32_COMP
+  call get_environment_variable('P2M_PRECISION', env)
+  if (trim(env) == 'fp64') cfg%precision = P2M_PREC_FP64
+  if (trim(env) == 'plain') cfg%precision = P2M_PREC_PLAIN_FP64
+  cfg%chunk_halos = 0
+  cfg%device = 0
+  cfg%flags = 0
+  rc = p2m_create(ctx, cfg)
+  call check(rc, 'p2m_create')
+
+  ! pks2map.f90:118-128 hard-coded cosmology; mmin of :84
+  run%omegam = 0.25d0
+  run%omegal = 0.75d0
+  run%h = 0.7d0
+  run%zmin = dble(zmin)
+  run%zmax = dble(zmax)
+  run%chihview = dble(chihview)
+  run%mmin = 1d13
+  run%flags = 0
+  call check(p2m_set_run(ctx, run), 'p2m_set_run')
+  call check(p2m_rofz(ctx, dble(zmin), chi_lo), 'p2m_rofz')
+  call check(p2m_rofz(ctx, dble(zmax), chi_hi), 'p2m_rofz')
+  write(*, 14) chi_lo / 1e3, chi_hi / 1e3
+14 format('cut_low, cut_high:  ', f4.2, 1x, f4.2, ' Gpc')
+
+  if (scramble == 1 .or. centre /= 0) then
+     call host_branches(halos, nh, real(chi_lo), real(chi_hi), chihview, scramble, centre)
+     run%flags = P2M_RUN_PRECUT
+     call check(p2m_set_run(ctx, run), 'p2m_set_run')
+  end if
+
+  call get_environment_variable('P2M_TILE', env)
+  if (len_trim(env) > 0) then
+     block
+       integer :: tw, twg
+       integer(c_int) :: p2m_set_tile_rc
+       interface
+          integer(c_int) function p2m_set_tile(ctx, w, wg) bind(C, name="p2m_set_tile")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: w, wg
+          end function p2m_set_tile
+       end interface
+       read(env, *, iostat=ios) tw, twg
+       if (ios == 0) then
+          p2m_set_tile_rc = p2m_set_tile(ctx, int(tw, c_int32_t), int(twg, c_int32_t))
+          call check(p2m_set_tile_rc, 'p2m_set_tile')
+       end if
+     end block
+  end if
+
+  do k = 1, ntab
+     call check(p2m_load_table(ctx, int(k - 1, c_int), hdr_i0, hdr_f0, tabs(:, k), ntv), 'p2m_load_table')
+  end do
+
+  ! ---------------------------------------------------------------- halo loop on the GPU
+  call check(p2m_paint(ctx, halos, nh, 0_c_int64_t), 'p2m_paint')
+  call check(p2m_get_stats(ctx, st), 'p2m_get_stats')
+  if (st%n_radius_err > 0) write(*, '(a,i0,a)') 'Warning: ', st%n_radius_err, &
+       ' halos with disc radius outside [0,pi] skipped'
+
+  write(*, 15) st%n_painted
+15 format('Number of halos in map:    ', i0)
+
+  do k = 1, ntab
+     call check(p2m_products(ctx, int(k - 1, c_int), moments, rms, dpdy), 'p2m_products')
+     write(*, 16) moments(1:3), moments(4:6)
+16   format('Flatsky moments:    ', 3(1pe9.3, 1x), /, 'Healpix moments:    ', 3(1pe9.3, 1x))
+     write(*, 17) rms(1), moments(1), rms(2), moments(4)
+17   format('Flatsky RMS:                ', 1pe9.3, /, 'Flatsky Mean:               ', 1pe8.2, /, &
+            'Healpix RMS:                ', 1pe9.3, /, 'Healpix Mean:               ', 1pe8.2, /)
+     if (ntab == 1) then
+        base = fileout
+     else
+        m = index(tables(k), '/', back=.true.)
+        p1 = len_trim(tables(k))
+        base = trim(fileout) // '_' // tables(k)(m + 1:p1)
+     end if
+     call check(p2m_write_outputs(ctx, int(k - 1, c_int), trim(base) // c_null_char, 1, 1, 1), 'p2m_write_outputs')
+  end do
+  write(*, *)
+
+  call p2m_destroy(ctx)
+
+contains
+
+  subroutine fail(msg, code)
+    character(len=*), intent(in) :: msg
+    integer, intent(in) :: code
+    write(0, '(a)') 'pks2map: ' // msg
+    select case (code)
+    case (2)
+       error stop 2
+    case (3)
+       error stop 3
+    case default
+       error stop 4
+    end select
+  end subroutine fail
+
+  subroutine check(code, what)
+    integer(c_int), intent(in) :: code
+    character(len=*), intent(in) :: what
+    if (code /= 0) then
+       write(0, '(a)') 'pks2map: ' // what // ': ' // p2m_strerror(code)
+       error stop 4
+    end if
+  end subroutine check
+
+  ! Radial cut + compaction (pks2map.f90:144-160), scramble (:162-187) and
+  ! centring on the largest halo (:189-262), in default real as the reference.
+  subroutine host_branches(h, n, cut_low, cut_high, chv, scr, cen)
+    real(c_float), intent(inout) :: h(:, :)
+    integer(c_int64_t), intent(inout) :: n
+    real, intent(in) :: cut_low, cut_high, chv
+    integer, intent(in) :: scr, cen
+    integer(c_int64_t) :: i, j
+    real :: xh, yh, zh, chih, rh, muran, phiran
+    real :: rthm, xm, ym, zm, rmaxtheta, rmaxphi, xml
+    integer :: seed(4)
+    double precision :: randnum
+    j = 0
+    do i = 1, n
+       xh = h(3, i) - chv
+       yh = h(2, i)
+       zh = h(1, i)
+       chih = sqrt(xh**2 + yh**2 + zh**2)
+       if (chih > cut_high .or. chih < cut_low) cycle
+       j = j + 1
+       h(:, j) = h(:, i)
+    end do
+    n = j
+    if (scr == 1) then
+       seed = 13579            ! every limb set to the same value (quirk Q10)
+       do i = 1, n
+          xh = h(1, i)
+          yh = h(2, i)
+          zh = h(3, i)
+          chih = sqrt(xh**2 + yh**2 + zh**2)
+          call ranf(seed, randnum)
+          muran = 2 * randnum - 1
+          call ranf(seed, randnum)
+          phiran = 2 * 3.14159 * randnum
+          rh = sqrt(1 - muran**2) * chih
+          h(1, i) = rh * cos(phiran)
+          h(2, i) = rh * sin(phiran)
+          h(3, i) = muran * chih
+       end do
+    end if
+    if (cen /= 0 .and. n > 0) then
+       rthm = maxval(h(7, 1:n))
+       xm = -1e5
+       ym = -1e5
+       zm = -1e5
+       do i = 1, n
+          if (h(7, i) == rthm) then
+             xm = h(3, i)
+             ym = h(2, i)
+             zm = h(1, i)
+          end if
+       end do
+       if (xm < 0.0) h(3, 1:n) = -h(3, 1:n)
+       if (ym < 0.0) h(2, 1:n) = -h(2, 1:n)
+       if (zm < 0.0) h(1, 1:n) = -h(1, 1:n)
+       xm = abs(xm)
+       ym = abs(ym)
+       zm = abs(zm)
+       rmaxtheta = asin(zm / sqrt(xm**2 + zm**2))
+       xml = xm / abs(xm) * sqrt(xm**2 + zm**2)
+       rmaxphi = asin(ym / sqrt(ym**2 + xml**2))
+       do i = 1, n
+          xh = h(3, i) * cos(rmaxtheta) + h(1, i) * sin(rmaxtheta)
+          zh = -h(3, i) * sin(rmaxtheta) + h(1, i) * cos(rmaxtheta)
+          yh = -xh * sin(rmaxphi) + h(2, i) * cos(rmaxphi)
+          xh = xh * cos(rmaxphi) + h(2, i) * sin(rmaxphi)
+          h(3, i) = zh
+          h(2, i) = yh
+          h(1, i) = xh
+       end do
+    end if
+  end subroutine host_branches
+
+  ! 48-bit LCG in four 12-bit limbs (RandomField/random.f90:40-57, 264-286)
+  subroutine ranf(seed, r)
+    integer, intent(inout) :: seed(4)
+    double precision, intent(out) :: r
+    integer, parameter :: mult(4) = (/373, 3707, 1442, 647/)
+    integer :: j1, j2, j3, j4, k1, k2, k3, k4
+    r = real(seed(4)) / 4096.0d0 + real(seed(3)) / 16777216.0d0 + &
+        real(seed(2)) / 68719476736.0d0 + real(seed(1)) / 281474976710656.0d0
+    j1 = mult(1) * seed(1)
+    j2 = mult(1) * seed(2) + mult(2) * seed(1)
+    j3 = mult(1) * seed(3) + mult(2) * seed(2) + mult(3) * seed(1)
+    j4 = mult(1) * seed(4) + mult(2) * seed(3) + mult(3) * seed(2) + mult(4) * seed(1)
+    k1 = j1
+    k2 = j2 + k1 / 4096
+    k3 = j3 + k2 / 4096
+    k4 = j4 + k3 / 4096
+    seed(1) = mod(k1, 4096)
+    seed(2) = mod(k2, 4096)
+    seed(3) = mod(k3, 4096)
+    seed(4) = mod(k4, 4096)
+  end subroutine ranf
+
+end program pks2map

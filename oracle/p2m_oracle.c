@@ -329,7 +329,7 @@ double p2mo_angdist(const double v1[3], const double v2[3])
  * each ring then uses its own range (the reference would misalign rows). */
 int64_t p2mo_query_disc(int32_t nside, const double v0[3], double radius, int32_t *list, int64_t *n_gaps)
 {
-    if (radius < 0.0 || radius > HPX_PI) return -1;
+    if (!(radius >= 0.0 && radius <= HPX_PI)) return -1;
     double norm = M_SQRT_((v0[0] * v0[0] + v0[1] * v0[1]) + v0[2] * v0[2]);
     double z0 = v0[2] / norm;
     double rlat0 = M_ASIN_(z0);
@@ -384,9 +384,11 @@ int64_t p2mo_query_disc(int32_t nside, const double v0[3], double radius, int32_
         int64_t np = hi - lo;
         np = fmodulo(np, npr) + 1;
         if (np > npr) np = npr;
-        for (int64_t i = 0; i < np; ++i) {
-            int64_t ip = fmodulo(lo + i, npr);
-            list[nlist + i] = (int32_t)(npc - npr + ip);
+        if (list) {
+            for (int64_t i = 0; i < np; ++i) {
+                int64_t ip = fmodulo(lo + i, npr);
+                list[nlist + i] = (int32_t)(npc - npr + ip);
+            }
         }
         nlist += np;
     }
@@ -439,7 +441,6 @@ int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
     const double delta_pksc = 200.0;                                /* pksc.f90:8-9 */
     const double rmax = 4.0;                                        /* haloproject.f90:18 */
     const int32_t nside = run->nside;
-    const int64_t hpnpix = nside > 0 ? p2mo_nside2npix(nside) : 0;
     const int32_t npix = run->npix;
 
     int64_t n_cut = 0, n_painted = 0, n_list = 0, n_upd_hp = 0, n_upd_fs = 0, n_gaps = 0;
@@ -448,7 +449,7 @@ int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
 #pragma omp parallel num_threads(nth) reduction(+:n_cut,n_painted,n_list,n_upd_hp,n_upd_fs,n_gaps)
     {
         int32_t *list = NULL;
-        if (nside > 0) list = (int32_t *)malloc(sizeof(int32_t) * (size_t)hpnpix);
+        int64_t list_cap = 0;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < n; ++i) {
             const float *h = halos + 7 * i;
@@ -522,8 +523,16 @@ int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
             /* ---- projecthalo_healpix, haloproject.f90:38-79 ---- */
             if (nside > 0) {
                 double v1[3] = { uz, uy, ux };
-                int64_t nl = p2mo_query_disc(nside, v1, dtheta, list, &n_gaps);
+                /* count first, then fill a list of that size (the reference
+                 * allocates a full-sky list, pks2map.f90:115, quirk Q19) */
+                int64_t nl = p2mo_query_disc(nside, v1, dtheta, NULL, NULL);
                 if (nl < 0) continue;
+                if (nl > list_cap) {
+                    free(list);
+                    list_cap = nl + 1024;
+                    list = (int32_t *)malloc(sizeof(int32_t) * (size_t)list_cap);
+                }
+                nl = p2mo_query_disc(nside, v1, dtheta, list, &n_gaps);
                 n_list += nl;
                 for (int64_t q = 0; q < nl; ++q) {
                     int32_t jp = list[q];
@@ -716,4 +725,28 @@ double p2mo_ranf(int32_t seed[4])
     int32_t k1 = j1, k2 = j2 + k1 / 4096, k3 = j3 + k2 / 4096, k4 = j4 + k3 / 4096;
     seed[0] = k1 % 4096; seed[1] = k2 % 4096; seed[2] = k3 % 4096; seed[3] = k4 % 4096;
     return r;
+}
+
+/* host evaluation of the elementary functions the oracle uses (same op codes
+ * as the library's p2m_selftest_math) */
+int p2mo_math(int op, const double *x, const double *y, double *out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double a = x[i], b = y ? y[i] : 0.0, r = 0.0;
+        switch (op) {
+        case 0: r = M_SQRT_(a); break;
+        case 1: r = M_SIN_(a); break;
+        case 2: r = M_COS_(a); break;
+        case 3: r = M_TAN_(a); break;
+        case 4: r = M_ASIN_(a); break;
+        case 5: r = M_ACOS_(a); break;
+        case 6: r = M_ATAN2_(a, b); break;
+        case 7: r = M_LOG_(a); break;
+        case 8: r = M_EXP_(a); break;
+        case 9: r = M_POW_(a, b); break;
+        default: return -1;
+        }
+        out[i] = r;
+    }
+    return 0;
 }

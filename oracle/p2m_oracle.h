@@ -103,6 +103,10 @@ int p2mo_make_maptable(int32_t model, double nu, double alpha_cmb,
                        int64_t node_lo, int64_t node_hi,
                        int32_t hdr_i[3], float hdr_f[6], float *table, int32_t nthreads);
 
+/* elementary functions as the oracle evaluates them (op codes: 0 sqrt, 1 sin,
+ * 2 cos, 3 tan, 4 asin, 5 acos, 6 atan2(x,y), 7 log, 8 exp, 9 pow(x,y)) */
+int p2mo_math(int op, const double *x, const double *y, double *out, int64_t n);
+
 /* 48-bit LCG of RandomField/random.f90:40-57 */
 double p2mo_ranf(int32_t seed[4]);
 

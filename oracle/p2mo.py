@@ -88,6 +88,7 @@ def lib(flavour="portable"):
     L.p2mo_halo_hash.restype = C.c_uint64
     L.p2mo_make_maptable.argtypes = [i32, d, d, i32, i32, i32, i64, i64, C.POINTER(i32), C.POINTER(C.c_float),
                                      C.POINTER(C.c_float), i32]
+    L.p2mo_math.argtypes = [C.c_int, C.POINTER(d), C.POINTER(d), C.POINTER(d), i64]
     L.p2mo_ranf.argtypes = [C.POINTER(i32)]
     L.p2mo_ranf.restype = d
     _LIBS[name] = L
@@ -220,3 +221,17 @@ def ranf(seed):
     s = (C.c_int32 * 4)(*seed)
     r = lib().p2mo_ranf(s)
     return r, list(s)
+
+
+MATH_OPS = ["sqrt", "sin", "cos", "tan", "asin", "acos", "atan2", "log", "exp", "pow"]
+
+
+def math(op, x, y=None, flavour="portable"):
+    L = lib(flavour)
+    x = np.ascontiguousarray(x, np.float64)
+    yy = None if y is None else np.ascontiguousarray(y, np.float64)
+    out = np.zeros_like(x)
+    rc = L.p2mo_math(MATH_OPS.index(op), _ptr(x, C.c_double), _ptr(yy, C.c_double), _ptr(out, C.c_double), x.size)
+    if rc:
+        raise ValueError(op)
+    return out
