@@ -502,7 +502,8 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, bool allow_tune)
             t.tab_flat[m] = c->d_flat[m]; t.tab_prof[m] = c->d_prof[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
         }
         t.count = c->d_count; t.hash = c->d_hash; t.counters = pc;
-        int grid = (int)std::min<int64_t>(c->n_tiles, 1 << 30);
+        /* bounded persistent grid: each workgroup strides over tiles */
+        int grid = (int)std::min<int64_t>(c->n_tiles, 8192);
         if (p2m_launch_tile_paint(t, mode, c->cfg.n_maps, grid, c->wg, c->stream)) return P2M_E_INVALID;
         HIPCHK(hipGetLastError());
     } else {

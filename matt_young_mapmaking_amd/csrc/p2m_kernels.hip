@@ -68,6 +68,21 @@ __device__ static inline void wave_count(unsigned long long *counter, unsigned l
     if (__lane_id() == 0 && v) atomicAdd(counter, v);
 }
 
+/* sum v over the workgroup, one atomic per workgroup; every thread calls */
+__device__ static inline void block_count(unsigned long long *counter, unsigned long long v, unsigned long long *lds16)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (__lane_id() == 0) lds16[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += lds16[k];
+        if (t) atomicAdd(counter, t);
+    }
+}
+
 /* ===================================================================== */
 /* decode                                                                 */
 /* ===================================================================== */
@@ -76,8 +91,13 @@ __global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
 {
     const CosmoDev &c = A.c;
     const TabGeom &g = A.g;
+    __shared__ unsigned long long lds16[16];
+    __shared__ unsigned int wcnt[4];
+    __shared__ unsigned long long base_s;
+    unsigned long long tot_cut = 0, tot_rad = 0, tot_painted = 0;
+    for (int64_t i0 = blockIdx.x * 256LL; i0 < A.n; i0 += (int64_t)gridDim.x * 256) {
     unsigned long long n_cut = 0, n_rad = 0;
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = i0 + threadIdx.x;
     bool painted = false;
     HaloRec h;
     if (i < A.n) {
@@ -177,17 +197,36 @@ __global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
             }
         }
     }
-    unsigned long long slot = wave_append(A.n_out, painted);
-    if (painted && slot < (unsigned long long)A.cap) A.out[slot] = h;
-    wave_count(&A.counters[0], n_cut);
-    wave_count(&A.counters[2], n_rad);
-    wave_count(&A.counters[1], painted ? 1ULL : 0ULL);
+    /* block-level append: one atomic per 256 halos */
+    unsigned long long mask = __ballot(painted);
+    int lane = __lane_id(), wv = threadIdx.x >> 6;
+    if (lane == 0) wcnt[wv] = (unsigned int)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        base_s = t ? atomicAdd(A.n_out, (unsigned long long)t) : 0ULL;
+    }
+    __syncthreads();
+    if (painted) {
+        unsigned long long off = base_s;
+        for (int k = 0; k < wv; ++k) off += wcnt[k];
+        unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+        off += (unsigned long long)__popcll(below);
+        if (off < (unsigned long long)A.cap) A.out[off] = h;
+    }
+    tot_cut += n_cut; tot_rad += n_rad; tot_painted += painted ? 1ULL : 0ULL;
+    __syncthreads();
+    }
+    block_count(&A.counters[0], tot_cut, lds16);
+    block_count(&A.counters[2], tot_rad, lds16);
+    block_count(&A.counters[1], tot_painted, lds16);
 }
 
 void p2m_launch_decode(const DecodeArgs &a, hipStream_t s)
 {
     if (a.n <= 0) return;
     int64_t g = (a.n + 255) / 256;
+    if (g > 8192) g = 8192;          /* grid-stride: bounded number of append atomics */
     hipLaunchKernelGGL(k_decode, dim3((unsigned)g), dim3(256), 0, s, a);
 }
 
@@ -355,25 +394,24 @@ __device__ static inline void for_each_piece(const HaloRec &h, const BinArgs &B,
 
 __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ unsigned long long lds16[16];
     unsigned long long n_list = 0, n_gaps = 0, n_seg = 0;
-    if (i < B.n_halo) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
         HaloRec h = B.halos[i];
         for_each_piece(h, B, [&](int32_t, int32_t tile, int32_t, int32_t) {
             atomicAdd(&B.tile_cnt[tile], 1u);
             n_seg++;
         }, n_list, n_gaps);
     }
-    wave_count(&B.counters[0], n_list);
-    wave_count(&B.counters[3], n_gaps);
-    wave_count(&B.counters[4], n_seg);
+    block_count(&B.counters[0], n_list, lds16);
+    block_count(&B.counters[3], n_gaps, lds16);
+    block_count(&B.counters[4], n_seg, lds16);
 }
 
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     unsigned long long n_list = 0, n_gaps = 0;
-    if (i < B.n_halo) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
         HaloRec h = B.halos[i];
         for_each_piece(h, B, [&](int32_t, int32_t tile, int32_t lo, int32_t len) {
             uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
@@ -383,15 +421,16 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
     }
 }
 
+static unsigned bin_grid(int64_t n) { int64_t g = (n + 255) / 256; return (unsigned)(g > 8192 ? 8192 : g); }
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s)
 {
     if (b.n_halo <= 0) return;
-    hipLaunchKernelGGL(k_seg_count, dim3((unsigned)((b.n_halo + 255) / 256)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_seg_count, dim3(bin_grid(b.n_halo)), dim3(256), 0, s, b);
 }
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s)
 {
     if (b.n_halo <= 0) return;
-    hipLaunchKernelGGL(k_seg_emit, dim3((unsigned)((b.n_halo + 255) / 256)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_seg_emit, dim3(bin_grid(b.n_halo)), dim3(256), 0, s, b);
 }
 
 /* exclusive scan of tile counts (uint32) into offsets (uint64) */
@@ -649,8 +688,9 @@ __global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
         }
         __syncthreads();
     }
-    wave_count(&A.counters[1], n_upd);
-    wave_count(&A.counters[5], n_fb);
+    __shared__ unsigned long long lds16[16];
+    block_count(&A.counters[1], n_upd, lds16);
+    block_count(&A.counters[5], n_fb, lds16);
 }
 
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership)

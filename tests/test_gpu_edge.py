@@ -1,0 +1,118 @@
+"""Edge cases on the GPU: empty and all-cut catalogues, discs centred on the
+poles (the a == 0 branch of discphirange_at_z), discs covering the pole or
+most of the sky, halos whose disc radius is invalid (r >= chi, NaN), the kSZ
+profile, and the drop-in Fortran executable end to end."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import p2mo
+from matt_young_mapmaking_amd import abi, catalog, formats
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def table():
+    return abi.make_maptable(model=1, nu=148.0, alpha=0.0)
+
+
+def paint_both(h, table, nside=256, npix=128, prec=abi.PREC_FP64, profile=1):
+    o = p2mo.Oracle(*table)
+    ref = o.paint(h, nside=nside, npix=npix, nthreads=4, profile=profile)
+    with abi.Painter(nside=nside, npix_fs=npix, precision=prec, membership=True, profile=profile) as p:
+        p.set_run()
+        p.load_table(0, *table)
+        p.paint(h)
+        hp, fs = p.maps()
+        mem = p.membership()
+        st = p.stats()
+    return ref, hp, fs, mem, st
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_PLAIN_FP64, abi.PREC_FP64, abi.PREC_FP32_COMP])
+def test_empty_and_all_cut(table, prec):
+    print(f"\n[gpu edge] empty precision={prec}", flush=True)
+    for h in (np.zeros((0, 7), np.float32), np.array([[0, 0, 50.0, 0, 0, 0, 5.0]], np.float32)):
+        with abi.Painter(nside=64, npix_fs=32, precision=prec, membership=True) as p:
+            p.set_run()
+            p.load_table(0, *table)
+            p.paint(h)
+            hp, fs = p.maps()
+            hc, hh, fc, fh = p.membership()
+        assert not hp.any() and not fs.any() and not hc.any() and not fc.any()
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_PLAIN_FP64, abi.PREC_FP64, abi.PREC_FP32_COMP])
+def test_poles_and_large_discs(table, prec):
+    print(f"\n[gpu edge] poles precision={prec}", flush=True)
+    h = np.array([
+        [0.0, 0.0, 300.0, 0, 0, 0, 12.0],      # on the painter x axis (flat-sky centre)
+        [300.0, 0.0, 0.0, 0, 0, 0, 12.0],      # hpvec z = pos1/chi = +1: north pole
+        [-300.0, 0.0, 0.0, 0, 0, 0, 12.0],     # south pole
+        [280.0, 30.0, 0.0, 0, 0, 0, 30.0],     # disc over the pole
+        [220.0, 0.0, 0.0, 0, 0, 0, 60.0],      # very large disc
+    ], np.float32)
+    ref, hp, fs, (hc, hh, fc, fh), st = paint_both(h, table, nside=512, npix=128, prec=prec)
+    assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+    assert np.array_equal(fc, ref["fs_count"])
+    nz = ref["hp"] != 0
+    tol = 1e-6 if prec == abi.PREC_FP32_COMP else 1e-12
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= tol
+
+
+def test_invalid_radius_is_counted_and_skipped(table):
+    # rth so large that rh > chi: asin(rh/chi) is undefined (quirk Q16)
+    h = np.array([[0, 0, 220.0, 0, 0, 0, 5000.0],
+                  [0, 0, 1500.0, 0, 0, 0, 8.0],
+                  [np.nan, 0, 1500.0, 0, 0, 0, 8.0]], np.float32)
+    with abi.Painter(nside=256, npix_fs=64, precision=abi.PREC_FP32_COMP, membership=True) as p:
+        p.set_run()
+        p.load_table(0, *table)
+        p.paint(h)
+        st = p.stats()
+        hc = p.membership()[0]
+    assert st["n_radius_err"] == 2
+    assert st["n_painted"] == 1
+    o = p2mo.Oracle(*table)
+    ref = o.paint(h[1:2], nside=256, npix=64)
+    assert np.array_equal(hc, ref["hp_count"])
+
+
+def test_ksz_profile(table):
+    h = catalog.config_catalog("C2", n=20000)
+    ref, hp, fs, (hc, hh, fc, fh), st = paint_both(h, table, nside=512, npix=128, prec=abi.PREC_FP64, profile=2)
+    assert np.array_equal(hh, ref["hp_hash"])
+    nz = ref["hp"] != 0
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= 1e-12
+    assert (hp > 0).any() and (hp < 0).any()      # -v_r/c takes both signs
+
+
+def test_fortran_pks2map_end_to_end(table, tmp_path):
+    exe = os.path.join(ROOT, "matt_young_mapmaking_amd", "bin", "pks2map")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built")
+    h = catalog.config_catalog("C1")
+    formats.write_pksc(tmp_path / "cat.pksc", h)
+    formats.write_table(tmp_path / "tab_148", *table)
+    r = subprocess.run([exe, "cat.pksc", "out", "tab_148", "0.05", "1024", "0", "0", "512", "10", "1.2", "0", "1", "1"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Number of halos in map:    17299" in r.stdout
+    assert "Healpix moments:" in r.stdout and "Flatsky RMS:" in r.stdout
+    # the executable parses CLI reals as float32, as the reference does
+    o = p2mo.Oracle(*table)
+    ref = o.paint(h, nside=1024, npix=512, zmin=float(np.float32(0.05)), zmax=float(np.float32(1.2)),
+                  fov_deg=10.0, nthreads=8)
+    ns, order, hp, _ = formats.read_healpix_fits(str(tmp_path / "out_hp.fits"))
+    nz = ref["hp"] != 0
+    assert np.count_nonzero((hp != 0) != nz) == 0
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) < 1e-6
+    n, fov, fs = formats.read_fsmap(str(tmp_path / "out_fs.map"))
+    nzf = ref["fs"] != 0
+    assert np.max(np.abs(fs[nzf] / ref["fs"][nzf] - 1)) < 1e-6
+    assert os.path.getsize(tmp_path / "out_dp.bin") == 8 + 40000

@@ -1,0 +1,143 @@
+"""GPU checks of the building blocks through the C ABI:
+
+  * the portable FP64 functions give the same bits on gfx950 as on the host
+    (sqrt included: the device expansion must be correctly rounded);
+  * make_maptable on the GPU reproduces the oracle's table bit for bit on
+    sampled grid nodes, for several models/frequencies;
+  * products (moments, RMS, dP/dy) and writers against numpy on the fetched maps;
+  * multi-map painting equals single-map painting per table;
+  * chunked decode, tile shapes and repeated runs give identical membership.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import p2mo
+from matt_young_mapmaking_amd import abi, catalog, formats
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+
+
+@pytest.mark.parametrize("op,gen", [
+    ("sqrt", lambda n: np.concatenate([np.exp(RNG.uniform(-60, 60, n)), np.arange(1, 20001, dtype=np.float64) ** 2,
+                                       np.nextafter(np.arange(1, 20001, dtype=np.float64) ** 2, 0)])),
+    ("sin", lambda n: RNG.uniform(-7, 7, n)), ("cos", lambda n: RNG.uniform(-7, 7, n)),
+    ("tan", lambda n: RNG.uniform(-1.5, 1.5, n)), ("asin", lambda n: RNG.uniform(-1, 1, n)),
+    ("acos", lambda n: RNG.uniform(-1, 1, n)), ("log", lambda n: np.exp(RNG.uniform(-60, 60, n))),
+    ("exp", lambda n: RNG.uniform(-700, 700, n))])
+def test_device_math_bits_equal_host(op, gen):
+    print(f"\n[gpu math] op={op}", flush=True)
+    x = gen(1_000_000)
+    dev = abi.selftest_math(op, x)
+    host = p2mo.math(op, x)
+    assert np.array_equal(dev.view(np.uint64), host.view(np.uint64))
+
+
+@pytest.mark.parametrize("op", ["atan2", "pow"])
+def test_device_math_binary_bits_equal_host(op):
+    print(f"\n[gpu math] op={op}", flush=True)
+    if op == "atan2":
+        x, y = RNG.uniform(-3, 3, 1_000_000), RNG.uniform(-3, 3, 1_000_000)
+    else:
+        x, y = np.exp(RNG.uniform(-30, 30, 1_000_000)), RNG.uniform(-3, 3, 1_000_000)
+    assert np.array_equal(abi.selftest_math(op, x, y).view(np.uint64), p2mo.math(op, x, y).view(np.uint64))
+
+
+@pytest.mark.parametrize("model,nu,alpha", [(1, 148.0, 0.0), (1, 353.0, 0.1), (2, 0.0, 0.0), (3, 545.0, 0.0)])
+def test_make_maptable_matches_oracle(model, nu, alpha):
+    print(f"\n[gpu maptable] model={model} nu={nu} alpha={alpha}", flush=True)
+    hi, hf, t = abi.make_maptable(model=model, nu=nu, alpha=alpha)
+    nodes = RNG.choice(100 * 100, 12, replace=False)
+    for nd in nodes:
+        a, b, ref = p2mo.make_maptable(model=model, nu=nu, alpha=alpha, nodes=(int(nd), int(nd) + 1), nthreads=4)
+        assert np.array_equal(a, hi) and np.array_equal(b, hf)
+        lo = 2 * 200 * int(nd)
+        assert np.array_equal(t[lo:lo + 400].view(np.uint32), ref[lo:lo + 400].view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def table():
+    return abi.make_maptable(model=1, nu=148.0, alpha=0.0)
+
+
+def test_products_and_writers(table, tmp_path):
+    h = catalog.config_catalog("C1")
+    with abi.Painter(nside=1024, npix_fs=512, precision=abi.PREC_FP32_COMP) as p:
+        p.set_run()
+        p.load_table(0, *table)
+        p.paint(h)
+        hp32, fs32 = p.maps(dtype=np.float32)
+        mom, rms, dp = p.products()
+        base = str(tmp_path / "out")
+        p.write_outputs(base)
+    hp = hp32.astype(np.float64)
+    fs = fs32.astype(np.float64)
+    assert mom[3] == pytest.approx(hp.mean(), rel=1e-12)
+    assert mom[4] == pytest.approx((hp ** 2).mean(), rel=1e-12)
+    assert mom[5] == pytest.approx((hp ** 3).mean(), rel=1e-12)
+    assert mom[0] == pytest.approx(fs.mean(), rel=1e-12)
+    assert rms[1] == pytest.approx(np.sqrt(((hp - hp.mean()) ** 2).mean()), rel=1e-10)
+    # dP/dy (pks2map.f90:342-357): m = int(y/dy)+1 in float32, bins 1..10000, cumulative / npix
+    dy = np.float32(1e-4) / np.float32(10000)
+    u = hp32 / dy
+    m = np.trunc(u).astype(np.int64) + 1
+    ok = (m > 0) & (m <= 10000) & (u > -1)
+    cnt = np.bincount(m[ok] - 1, minlength=10000)
+    ref = np.cumsum((cnt.astype(np.float32) / np.float32(hp.size)).astype(np.float32), dtype=np.float32)
+    assert np.allclose(dp, ref, rtol=1e-6, atol=1e-9)
+    # files
+    n, fov, fsm = formats.read_fsmap(base + "_fs.map")
+    assert n == 512 and np.array_equal(fsm, fs32)
+    ns, order, hpm, _ = formats.read_healpix_fits(base + "_hp.fits")
+    assert ns == 1024 and order == "RING" and np.array_equal(hpm, hp32)
+    nd, mx, dpf = formats.read_dpdy(base + "_dp.bin")
+    assert nd == 10000 and np.array_equal(dpf, dp)
+
+
+def test_multimap_equals_single_maps(table):
+    tabs = [table] + [abi.make_maptable(model=1, nu=nu, alpha=a) for nu, a in ((200.0, 0), (353.0, 0.1))]
+    h = catalog.config_catalog("C3", n=100000)
+    singles = []
+    for t in tabs:
+        with abi.Painter(nside=2048, npix_fs=256, precision=abi.PREC_FP64) as p:
+            p.set_run()
+            p.load_table(0, *t)
+            p.paint(h)
+            singles.append(p.maps())
+    for prec, tol in ((abi.PREC_FP64, 1e-12), (abi.PREC_FP32_COMP, 1e-6)):
+        with abi.Painter(nside=2048, npix_fs=256, precision=prec, n_maps=3) as p:
+            p.set_run()
+            for k, t in enumerate(tabs):
+                p.load_table(k, *t)
+            p.paint(h)
+            for k in range(3):
+                hp, fs = p.maps(slot=k)
+                nz = singles[k][0] != 0
+                assert np.max(np.abs(hp[nz] / singles[k][0][nz] - 1)) <= tol
+                assert np.count_nonzero((hp != 0) != nz) == 0
+                nzf = singles[k][1] != 0
+                assert np.max(np.abs(fs[nzf] / singles[k][1][nzf] - 1)) <= 1e-12
+
+
+def test_chunking_tiles_and_repeatability(table):
+    h = catalog.config_catalog("C3", n=300000)
+    ref = None
+    for chunk, tile in ((0, None), (1 << 16, (256, 256)), (70001, (2048, 512)), (0, (512, 128)), (0, None)):
+        with abi.Painter(nside=4096, npix_fs=0, precision=abi.PREC_FP32_COMP, membership=True,
+                         chunk_halos=chunk) as p:
+            if tile:
+                p.set_tile(*tile)
+            p.set_run()
+            p.load_table(0, *table)
+            p.paint(h)
+            hp, _ = p.maps()
+            hc, hh, _, _ = p.membership()
+        if ref is None:
+            ref = (hp, hc, hh)
+            continue
+        assert np.array_equal(hc, ref[1]) and np.array_equal(hh, ref[2])
+        nz = ref[0] != 0
+        assert np.max(np.abs(hp[nz] / ref[0][nz] - 1)) < 2e-7
