@@ -43,7 +43,7 @@ namespace {
 const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
 
 struct Ctr {                 /* layout of the device counter block */
-    enum { N_OUT = 0, DEC = 1, PAINT = 8, TOTAL = 16 };
+    enum { N_OUT = 0, DEC = 1, N_FLAT = 5, PAINT = 8, TOTAL = 16 };
 };
 
 }  // namespace
@@ -66,6 +66,9 @@ struct p2m_ctx {
     float *d_prof[P2M_MAXMAPS] = {nullptr};
     int64_t n_flat = 0, n_prof = 0;
     double *d_inv_node2 = nullptr;
+    float *d_inv_node2f = nullptr;
+    uint4 *d_fb = nullptr;
+    int64_t fb_cap = 0;
     float fr_clamp = 0.f, u_k1 = 0.f, u_k0 = 0.f, u_max = 0.f;
     int64_t hpnpix = 0, fsnpix = 0;
     double *d_hp64[P2M_MAXMAPS] = {nullptr};
@@ -76,6 +79,11 @@ struct p2m_ctx {
     bool hp_zero = true;        /* next paint may store instead of accumulate */
     bool hp_valid = true;       /* device HEALPix arrays hold the map (false after a lazy clear) */
     HaloRec *d_halo = nullptr;
+    uint32_t *d_flat_idx = nullptr;
+    uint32_t *d_nr = nullptr;
+    unsigned long long *d_ring_off = nullptr;
+    uint2 *d_runs = nullptr;
+    int64_t runs_cap = 0;
     int64_t halo_cap = 0;
     unsigned long long *d_ctr = nullptr;
     int log2w = 0, wg = 0;
@@ -141,8 +149,8 @@ static int build_tiles(p2m_ctx *c, int log2w)
         acc += (npr + W - 1) / W;
     }
     c->n_tiles = acc;
-    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
-    c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_fill = nullptr; c->d_tile_off = nullptr; c->d_scan_tmp = nullptr;
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off);
+    c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_fill = nullptr; c->d_tile_off = nullptr;
     int rc = 0;
     rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
     rc |= dmalloc((void **)&c->d_tile_cnt, sizeof(uint32_t) * (c->n_tiles + 1));
@@ -151,8 +159,6 @@ static int build_tiles(p2m_ctx *c, int log2w)
     if (rc) return P2M_E_NOMEM;
     HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
     c->scan_tmp_bytes = 0;
-    if (p2m_scan_tiles(c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1, nullptr, &c->scan_tmp_bytes, c->stream)) return P2M_E_HIP;
-    if (dmalloc(&c->d_scan_tmp, c->scan_tmp_bytes + 256)) return P2M_E_NOMEM;
     c->log2w = log2w;
     return 0;
 }
@@ -235,7 +241,8 @@ void p2m_destroy(p2m_ctx *c)
         dfree(c->d_hp64[m]); dfree(c->d_hp32[m]); dfree(c->d_fs64[m]); dfree(c->d_flat[m]); dfree(c->d_prof[m]);
     }
     dfree(c->d_count); dfree(c->d_hash); dfree(c->d_fs_count); dfree(c->d_fs_hash);
-    dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_halo); dfree(c->d_ctr);
+    dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
+    dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
     dfree(c->d_segs); dfree(c->d_stage);
     for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
@@ -376,8 +383,12 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     /* FP32 path constants: node radii, ln r -> cell coordinate */
     std::vector<double> inv(g.nrt + 2);
     for (int k = 0; k < g.nrt + 2; ++k) inv[k] = p2m_exp(-2.0 * (g.lr0 + (double)k * g.drt));
+    std::vector<float> invf(g.nrt + 2);
+    for (int k = 0; k < g.nrt + 2; ++k) invf[k] = (float)inv[k];
     if (!c->d_inv_node2 && dmalloc((void **)&c->d_inv_node2, sizeof(double) * (g.nrt + 2))) return P2M_E_NOMEM;
+    if (!c->d_inv_node2f && dmalloc((void **)&c->d_inv_node2f, sizeof(float) * (g.nrt + 2))) return P2M_E_NOMEM;
     HIPCHK(hipMemcpy(c->d_inv_node2, inv.data(), sizeof(double) * (g.nrt + 2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_inv_node2f, invf.data(), sizeof(float) * (g.nrt + 2), hipMemcpyHostToDevice));
     c->u_k1 = (float)(0.5 * 0.6931471805599453 / g.drt);   /* log2(r^2) -> (ln r)/drt */
     c->u_k0 = (float)(g.lr0 / g.drt);
     c->u_max = (float)(g.nrt - 1);
@@ -392,6 +403,7 @@ int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
     if (tile_w == 0 && tile_wg == 0) { c->tuned = false; return 0; }
     if (!is_pow2(tile_w) || tile_w < 64 || tile_w > 4096) return P2M_E_INVALID;
     if (tile_wg < 64 || tile_wg > 512 || tile_wg % 64) return P2M_E_INVALID;
+    if (c->cfg.precision == P2M_PREC_FP32_COMP && tile_wg > 256) return P2M_E_INVALID;
     int l = 0;
     while ((1 << l) < tile_w) ++l;
     if (p2m_tile_lds_bytes(c->cfg.precision == P2M_PREC_FP64 ? 1 : 2, c->cfg.n_maps, l, c->d_count != nullptr) > 160 * 1024)
@@ -407,12 +419,33 @@ int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
 }  /* extern "C" */
 
 /* ------------------------------------------------------------------ paint */
+/* exclusive scan of n uint32 counts into uint64 offsets, growing the temp storage */
+static int scan_counts(p2m_ctx *c, const uint32_t *cnt, unsigned long long *off, int64_t n)
+{
+    size_t need = 0;
+    if (p2m_scan_u32(cnt, off, n, nullptr, &need, c->stream)) return P2M_E_HIP;
+    if (need > c->scan_tmp_bytes || !c->d_scan_tmp) {
+        dfree(c->d_scan_tmp); c->d_scan_tmp = nullptr; c->scan_tmp_bytes = 0;
+        if (dmalloc(&c->d_scan_tmp, need + 256)) return P2M_E_NOMEM;
+        c->scan_tmp_bytes = need + 256;
+    }
+    size_t bytes = c->scan_tmp_bytes;
+    if (p2m_scan_u32(cnt, off, n, c->d_scan_tmp, &bytes, c->stream)) return P2M_E_HIP;
+    return 0;
+}
+
 static int ensure_halo_cap(p2m_ctx *c, int64_t n)
 {
     if (n <= c->halo_cap) return 0;
     dfree(c->d_halo); c->d_halo = nullptr; c->halo_cap = 0;
+    dfree(c->d_flat_idx); c->d_flat_idx = nullptr;
+    dfree(c->d_nr); c->d_nr = nullptr;
+    dfree(c->d_ring_off); c->d_ring_off = nullptr;
     int64_t cap = std::max<int64_t>(n, 1024);
     if (dmalloc((void **)&c->d_halo, sizeof(HaloRec) * cap)) return P2M_E_NOMEM;
+    if (dmalloc((void **)&c->d_flat_idx, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
+    if (dmalloc((void **)&c->d_nr, sizeof(uint32_t) * (cap + 1))) return P2M_E_NOMEM;
+    if (dmalloc((void **)&c->d_ring_off, sizeof(unsigned long long) * (cap + 1))) return P2M_E_NOMEM;
     c->halo_cap = cap;
     return 0;
 }
@@ -430,20 +463,23 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
     a.dpix = c->cfg.npix_fs > 0 ? a.fov / (double)c->cfg.npix_fs : 1.0;
     a.out = c->d_halo; a.cap = c->halo_cap;
     a.n_out = c->d_ctr + Ctr::N_OUT;
+    a.flat_idx = c->d_flat_idx;
+    a.n_flat_out = c->d_ctr + Ctr::N_FLAT;
     a.counters = c->d_ctr + Ctr::DEC;
     p2m_launch_decode(a, c->stream);
     return hipGetLastError() == hipSuccess ? 0 : P2M_E_HIP;
 }
 
-static int autotune(p2m_ctx *c, int64_t n_halo);
+static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat);
 
-static int paint_halos(p2m_ctx *c, int64_t n_halo, bool allow_tune)
+static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tune)
 {
     const int mode = c->cfg.precision;
     unsigned long long *pc = c->d_ctr + Ctr::PAINT;
     PaintArgs pa;
     memset(&pa, 0, sizeof(pa));
     pa.halos = c->d_halo; pa.n_halo = n_halo; pa.nside = c->cfg.nside; pa.npix_fs = c->cfg.npix_fs;
+    pa.flat_idx = c->d_flat_idx; pa.n_flat_dev = c->d_ctr + Ctr::N_FLAT; pa.n_flat_host = n_flat;
     pa.n_maps = c->cfg.n_maps; pa.profile = c->cfg.profile;
     pa.fov = ((c->cfg.fov_deg / 360.0) * 2.0) * c->mypi;
     pa.dpix = c->cfg.npix_fs > 0 ? pa.fov / (double)c->cfg.npix_fs : 1.0;
@@ -466,21 +502,45 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, bool allow_tune)
     }
     if (c->cfg.nside > 0) {
         if (!c->tuned) {
-            if (allow_tune) { int rc = autotune(c, n_halo); if (rc) return rc; }
+            if (allow_tune) { int rc = autotune(c, n_halo, n_flat); if (rc) return rc; }
             else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
         }
         BinArgs b;
+        memset(&b, 0, sizeof(b));
         b.halos = c->d_halo; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
+        b.nr = c->d_nr; b.ring_off = c->d_ring_off;
         b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt; b.tile_fill = c->d_tile_fill;
         b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc;
+        /* (halo, ring) item space */
+        p2m_launch_ring_counts(b, c->stream);
+        { int rc = scan_counts(c, c->d_nr, c->d_ring_off, n_halo + 1); if (rc) return rc; }
+        unsigned long long n_items = 0;
+        HIPCHK(hipMemcpyAsync(&n_items, c->d_ring_off + n_halo, sizeof(n_items), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if ((int64_t)n_items > c->runs_cap) {
+            dfree(c->d_runs); c->d_runs = nullptr; c->runs_cap = 0;
+            int64_t cap = (int64_t)(n_items * 1.25) + 1024;
+            if (dmalloc((void **)&c->d_runs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
+            c->runs_cap = cap;
+        }
+        b.n_items = n_items; b.runs = c->d_runs;
         p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_fill_u32(c->d_tile_fill, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_seg_count(b, c->stream);
-        if (p2m_scan_tiles(c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1, c->d_scan_tmp, &c->scan_tmp_bytes, c->stream))
-            return P2M_E_HIP;
-        unsigned long long nseg = 0;
+        { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
+        unsigned long long nseg = 0, nlisted = 0;
         HIPCHK(hipMemcpyAsync(&nseg, c->d_tile_off + c->n_tiles, sizeof(nseg), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(&nlisted, pc + 0, sizeof(nlisted), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (mode == P2M_PREC_FP32_COMP) {
+            /* deferred FP64 pixels: ~5e-4 of the listed pixels in practice */
+            int64_t need = (int64_t)(nlisted / 16) + (1 << 20);
+            if (need > c->fb_cap) {
+                dfree(c->d_fb); c->d_fb = nullptr; c->fb_cap = 0;
+                if (dmalloc((void **)&c->d_fb, sizeof(uint4) * need)) return P2M_E_NOMEM;
+                c->fb_cap = need;
+            }
+        }
         if ((int64_t)nseg > c->seg_cap) {
             dfree(c->d_segs); c->d_segs = nullptr; c->seg_cap = 0;
             int64_t cap = (int64_t)(nseg * 1.25) + 1024;
@@ -498,6 +558,9 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, bool allow_tune)
         t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
         t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
         t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
+        t.band_c = 4e-6f;
+        t.inv_node2f = c->d_inv_node2f;
+        t.fb = c->d_fb; t.fb_count = pc + 6; t.fb_cap = c->fb_cap;
         for (int m = 0; m < c->cfg.n_maps; ++m) {
             t.tab_flat[m] = c->d_flat[m]; t.tab_prof[m] = c->d_prof[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
         }
@@ -506,6 +569,16 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, bool allow_tune)
         int grid = (int)std::min<int64_t>(c->n_tiles, 8192);
         if (p2m_launch_tile_paint(t, mode, c->cfg.n_maps, grid, c->wg, c->stream)) return P2M_E_INVALID;
         HIPCHK(hipGetLastError());
+        if (mode == P2M_PREC_FP32_COMP) {
+            FallbackArgs f;
+            memset(&f, 0, sizeof(f));
+            f.halos = c->d_halo; f.fb = c->d_fb; f.fb_count = pc + 6; f.fb_cap = c->fb_cap;
+            f.nside = c->cfg.nside; f.n_maps = c->cfg.n_maps; f.profile = c->cfg.profile;
+            f.g = c->g; f.n_flat = c->n_flat;
+            for (int m = 0; m < c->cfg.n_maps; ++m) { f.tab_flat[m] = c->d_flat[m]; f.map32[m] = c->d_hp32[m]; }
+            f.count = c->d_count; f.hash = c->d_hash; f.counters = pc;
+            if (p2m_launch_fallback(f, (int64_t)(nlisted / 256) + 1024, c->stream)) return P2M_E_HIP;
+        }
     } else {
         HIPCHK(hipEventRecord(c->ev[3], c->stream));
     }
@@ -533,6 +606,11 @@ static int finish_stats(p2m_ctx *c)
     s.n_ring_gaps = (int64_t)h[Ctr::PAINT + 3];
     s.n_segments = (int64_t)h[Ctr::PAINT + 4];
     s.n_fallback = (int64_t)h[Ctr::PAINT + 5];
+    if (c->cfg.precision == P2M_PREC_FP32_COMP && (int64_t)h[Ctr::PAINT + 6] > c->fb_cap) {
+        snprintf(g_hip_detail, sizeof(g_hip_detail), "deferred-pixel list overflow (%llu > %lld)",
+                 (unsigned long long)h[Ctr::PAINT + 6], (long long)c->fb_cap);
+        return P2M_E_NOMEM;
+    }
     s.cut_low = c->cut_low; s.cut_high = c->cut_high;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); s.ms_total = ms;
@@ -575,11 +653,12 @@ static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n,
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    unsigned long long n_out = 0;
-    HIPCHK(hipMemcpyAsync(&n_out, c->d_ctr + Ctr::N_OUT, sizeof(n_out), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long hc[Ctr::PAINT];
+    HIPCHK(hipMemcpyAsync(hc, c->d_ctr, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    int64_t n_halo = (int64_t)std::min<unsigned long long>(n_out, (unsigned long long)c->halo_cap);
-    rc = paint_halos(c, n_halo, allow_tune);
+    int64_t n_halo = (int64_t)std::min<unsigned long long>(hc[Ctr::N_OUT], (unsigned long long)c->halo_cap);
+    int64_t n_flat = (int64_t)std::min<unsigned long long>(hc[Ctr::N_FLAT], (unsigned long long)c->halo_cap);
+    rc = paint_halos(c, n_halo, n_flat, allow_tune);
     if (rc) return rc;
     c->hp_valid = true;
     c->st.n_in = n;
@@ -659,9 +738,9 @@ int p2m_fetch_membership(p2m_ctx *c, uint32_t *hc, uint64_t *hh, uint32_t *fc, u
 /* ------------------------------------------------------------------ autotuner */
 /* Times the binning + tile paint for each candidate tile shape on the
  * halos already decoded, keeps the fastest, and restores the maps. */
-static int autotune(p2m_ctx *c, int64_t n_halo)
+static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat)
 {
-    static const int cand[][2] = {{9, 256}, {10, 256}, {10, 512}, {11, 256}, {11, 512}};
+    static const int cand[][2] = {{9, 256}, {10, 256}, {11, 256}, {11, 128}, {12, 256}};
     int mode_ld = c->cfg.precision == P2M_PREC_FP64 ? 1 : 2;
     int best_l = 10, best_wg = 256;
     /* small problems: the default shape */
@@ -685,7 +764,7 @@ static int autotune(p2m_ctx *c, int64_t n_halo)
         int rc = build_tiles(c, cd[0]); if (rc) return rc;
         c->wg = cd[1]; c->tuned = true;
         c->hp_zero = true;
-        rc = paint_halos(c, n_halo, false);
+        rc = paint_halos(c, n_halo, n_flat, false);
         if (rc == P2M_E_INVALID) { (void)p2m_clear(c); continue; }
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->stream));

@@ -21,12 +21,17 @@ struct DecodeArgs {
     HaloRec *out;
     int64_t cap;
     unsigned long long *n_out;   /* append counter */
+    uint32_t *flat_idx;          /* slots of halos the flat-sky painter covers */
+    unsigned long long *n_flat_out;
     unsigned long long *counters;/* [0] passed radial cut, [1] painted, [2] radius errors */
 };
 
 struct PaintArgs {
     const HaloRec *halos;
     int64_t n_halo;
+    const uint32_t *flat_idx;     /* flat-sky halo list (tiled painters) */
+    const unsigned long long *n_flat_dev;
+    int64_t n_flat_host;          /* its length, known on the host */
     int32_t nside, npix_fs, n_maps, profile;
     double fov, dpix;
     TabGeom g;
@@ -45,6 +50,10 @@ struct BinArgs {
     const HaloRec *halos;
     int64_t n_halo;
     int32_t nside, log2w;
+    uint32_t *nr;                 /* rings per halo (n_halo + 1 entries, last 0) */
+    const unsigned long long *ring_off;  /* exclusive scan of nr */
+    unsigned long long n_items;   /* total (halo, ring) items */
+    uint2 *runs;                  /* per item: (first in-ring pixel, pixel count) */
     const int32_t *tile_base;     /* first tile of ring r+1 */
     uint32_t *tile_cnt;
     uint32_t *tile_fill;
@@ -65,12 +74,32 @@ struct TileArgs {
     const float *tab_prof[P2M_MAXMAPS];
     int64_t n_flat;
     const double *inv_node2;      /* exp(-2 (lr0 + k drt)), k = 0..nrt */
+    const float *inv_node2f;      /* the same in float */
     float u_k1, u_k0, u_max, band, fr_clamp;
+    float band_c;                 /* relative chord^2 band decided in FP64 */
     double *map64[P2M_MAXMAPS];
     float *map32[P2M_MAXMAPS];
     uint32_t *count;
     unsigned long long *hash;
+    uint4 *fb;                    /* deferred FP64 pixels: (ring, ip, halo, 0) */
+    unsigned long long *fb_count;
+    int64_t fb_cap;
     unsigned long long *counters; /* [1] hp updates, [5] FP64 fallbacks */
+};
+
+struct FallbackArgs {
+    const HaloRec *halos;
+    const uint4 *fb;
+    const unsigned long long *fb_count;
+    int64_t fb_cap;
+    int32_t nside, n_maps, profile;
+    TabGeom g;
+    const float *tab_flat[P2M_MAXMAPS];
+    int64_t n_flat;
+    float *map32[P2M_MAXMAPS];
+    uint32_t *count;
+    unsigned long long *hash;
+    unsigned long long *counters; /* [1] hp updates */
 };
 
 void p2m_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
@@ -80,11 +109,13 @@ void p2m_launch_fill_f32(float *p, int64_t n, hipStream_t s);
 void p2m_launch_decode(const DecodeArgs &a, hipStream_t s);
 void p2m_launch_paint_plain_hp(const PaintArgs &a, hipStream_t s);
 void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s);
+void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
-int p2m_scan_tiles(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
+int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership);
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);
+int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s);
 void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double mean, double *out,
                         unsigned long long *hist, int nb, float dy, hipStream_t s);
 void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s);

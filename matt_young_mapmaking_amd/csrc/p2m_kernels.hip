@@ -212,7 +212,13 @@ __global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
         for (int k = 0; k < wv; ++k) off += wcnt[k];
         unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
         off += (unsigned long long)__popcll(below);
-        if (off < (unsigned long long)A.cap) A.out[off] = h;
+        if (off < (unsigned long long)A.cap) {
+            A.out[off] = h;
+            if (h.flat) {
+                unsigned long long fs = atomicAdd(A.n_flat_out, 1ULL);
+                A.flat_idx[fs] = (uint32_t)off;
+            }
+        }
     }
     tot_cut += n_cut; tot_rad += n_rad; tot_painted += painted ? 1ULL : 0ULL;
     __syncthreads();
@@ -322,8 +328,9 @@ __global__ void __launch_bounds__(256) k_paint_fs(PaintArgs A, int plain)
         wave_count(&A.counters[2], n_upd);
         return;
     }
-    if (hidx < A.n_halo && A.halos[hidx].flat) {
-        const HaloRec &h = A.halos[hidx];
+    const unsigned long long nfl = *A.n_flat_dev;
+    if ((unsigned long long)hidx < nfl) {
+        const HaloRec &h = A.halos[A.flat_idx[hidx]];
         int nj = h.jmax - h.jmin + 1, nk = h.kmax - h.kmin + 1;
         int64_t tot = (int64_t)nj * nk;
         double hf = A.fov / 2.0;
@@ -352,93 +359,165 @@ void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s)
 {
     if (a.n_halo <= 0 || a.npix_fs <= 0) return;
     if (plain) hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_halo + 255) / 256)), dim3(256), 0, s, a, 1);
-    else hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_halo + 3) / 4)), dim3(256), 0, s, a, 0);
+    else if (a.n_flat_host > 0) hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_flat_host + 3) / 4)), dim3(256), 0, s, a, 0);
 }
 
 /* ===================================================================== */
 /* binning of (halo, ring) runs into ring tiles                           */
 /* ===================================================================== */
 
-/* visit the tile pieces of every ring run of halo h:
- *   f(ring, tile, lo_in_tile, len) */
+/* The binning works on the flattened (halo, ring) item space: item i
+ * belongs to halo h with ring_off[h] <= i < ring_off[h+1] and is ring
+ * irmin(h) + i - ring_off[h].  Every halo has at least one ring, so the 64
+ * consecutive items a wave handles span at most 64 halos; each lane finds
+ * its halo with a 6-step shuffle search over off[h0 .. h0+63]. */
 template <typename F>
-__device__ static inline void for_each_piece(const HaloRec &h, const BinArgs &B, F f, unsigned long long &n_list,
-                                             unsigned long long &n_gaps)
+__device__ static inline void wave_items(const unsigned long long *off, int64_t n_halo, unsigned long long total, F f)
 {
-    int last = 0, pend = 0;
-    for (int32_t ring = h.irmin; ring <= h.irmax; ++ring) {
-        int32_t npr, kshift; int64_t first;
-        ring_info(B.nside, ring, npr, kshift, first);
-        int32_t lo = 0;
-        int32_t np = disc_ring_run(h, B.nside, ring, npr, kshift, lo);
-        if (np == 0) { if (last) pend = 1; continue; }
-        if (pend) { n_gaps++; pend = 0; }
-        last = 1;
-        n_list += np;
-        int32_t tb = B.tile_base[ring - 1];
-        int32_t a0 = lo, b0 = lo + np;           /* [a0, b0) possibly wrapping */
-        for (int piece = 0; piece < 2; ++piece) {
-            int32_t a, b;
-            if (piece == 0) { a = a0; b = b0 < npr ? b0 : npr; }
-            else { if (b0 <= npr) break; a = 0; b = b0 - npr; }
-            int32_t c0 = a >> B.log2w, c1 = (b - 1) >> B.log2w;
-            for (int32_t col = c0; col <= c1; ++col) {
-                int32_t s = col << B.log2w;
-                int32_t pa = a > s ? a : s;
-                int32_t pb = b < s + (1 << B.log2w) ? b : s + (1 << B.log2w);
-                f(ring, tb + col, pa - s, pb - pa);
-            }
+    const int lane = __lane_id();
+    const unsigned long long wid = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) >> 6;
+    const unsigned long long nw = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
+    const unsigned long long per = (total + nw - 1) / nw;
+    const unsigned long long start = wid * per;
+    unsigned long long end = start + per;
+    if (end > total) end = total;
+    if (start >= end) return;
+    int64_t lo = 0, hi = n_halo - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= start) lo = mid; else hi = mid - 1;
+    }
+    int64_t h = lo;
+    for (unsigned long long c = start; c < end; c += 64) {
+        while (h + 1 < n_halo && off[h + 1] <= c) ++h;
+        const unsigned long long item = c + lane;
+        const int64_t hl = h + lane;
+        const unsigned long long my_off = hl <= n_halo ? off[hl] : ~0ULL;
+        int j = 0;
+        for (int step = 32; step > 0; step >>= 1) {
+            unsigned long long o = __shfl(my_off, j + step);
+            if (o <= item) j += step;
+        }
+        const unsigned long long hoff = __shfl(my_off, j);
+        if (item < end) f(item, h + j, item - hoff);
+        h += __shfl(j, 63);
+    }
+}
+
+/* tile pieces of one ring run [lo, lo+np) (wrapping at npr): f(tile, lo_in_tile, len) */
+template <typename F>
+__device__ static inline void run_pieces(int32_t lo, int32_t np, int32_t npr, int32_t tb, int log2w, F f)
+{
+    const int32_t a0 = lo, b0 = lo + np;
+    for (int piece = 0; piece < 2; ++piece) {
+        int32_t a, b;
+        if (piece == 0) { a = a0; b = b0 < npr ? b0 : npr; }
+        else { if (b0 <= npr) break; a = 0; b = b0 - npr; }
+        const int32_t c0 = a >> log2w, c1 = (b - 1) >> log2w;
+        for (int32_t col = c0; col <= c1; ++col) {
+            const int32_t s0 = col << log2w;
+            const int32_t pa = a > s0 ? a : s0;
+            const int32_t pb = b < s0 + (1 << log2w) ? b : s0 + (1 << log2w);
+            f(tb + col, pa - s0, pb - pa);
         }
     }
 }
 
+__global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t nr = 0;
+        if (i < B.n_halo) {
+            const HaloRec &h = B.halos[i];
+            nr = h.irmax >= h.irmin ? (uint32_t)(h.irmax - h.irmin + 1) : 1u;
+        }
+        B.nr[i] = nr;
+    }
+}
+
+/* per (halo, ring): the query_disc pixel run (stored for emit) and its tile counts */
 __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
 {
     __shared__ unsigned long long lds16[16];
-    unsigned long long n_list = 0, n_gaps = 0, n_seg = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
-        HaloRec h = B.halos[i];
-        for_each_piece(h, B, [&](int32_t, int32_t tile, int32_t, int32_t) {
-            atomicAdd(&B.tile_cnt[tile], 1u);
-            n_seg++;
-        }, n_list, n_gaps);
-    }
+    unsigned long long n_list = 0, n_seg = 0;
+    wave_items(B.ring_off, B.n_halo, B.n_items, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+        const HaloRec &h = B.halos[hi];
+        const int32_t ring = h.irmin + (int32_t)k;
+        int32_t lo = 0, np = 0;
+        int32_t npr, kshift; int64_t first;
+        if (ring <= h.irmax) {
+            ring_info(B.nside, ring, npr, kshift, first);
+            np = disc_ring_run(h, B.nside, ring, npr, kshift, lo);
+        }
+        B.runs[item] = make_uint2((uint32_t)lo, (uint32_t)np);
+        if (np > 0) {
+            n_list += (unsigned long long)np;
+            run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t, int32_t) {
+                atomicAdd(&B.tile_cnt[tile], 1u);
+                n_seg++;
+            });
+        }
+    });
     block_count(&B.counters[0], n_list, lds16);
-    block_count(&B.counters[3], n_gaps, lds16);
     block_count(&B.counters[4], n_seg, lds16);
+}
+
+/* empty rings between non-empty rings of one disc, counted once per gap
+ * (the row misalignment of discedge2fulldisc, quirk Q15) */
+__global__ void __launch_bounds__(256) k_ring_gaps(BinArgs B)
+{
+    __shared__ unsigned long long lds16[16];
+    unsigned long long n_gaps = 0;
+    wave_items(B.ring_off, B.n_halo, B.n_items, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+        if (k == 0 || B.runs[item].y != 0 || B.runs[item - 1].y == 0) return;
+        const unsigned long long endi = B.ring_off[hi + 1];
+        for (unsigned long long j = item + 1; j < endi; ++j)
+            if (B.runs[j].y != 0) { n_gaps++; break; }
+    });
+    block_count(&B.counters[3], n_gaps, lds16);
 }
 
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
-    unsigned long long n_list = 0, n_gaps = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
-        HaloRec h = B.halos[i];
-        for_each_piece(h, B, [&](int32_t, int32_t tile, int32_t lo, int32_t len) {
-            uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
-            uint64_t pos = B.tile_off[tile] + slot;
-            B.segs[pos] = make_uint2((uint32_t)i, ((uint32_t)lo << 16) | (uint32_t)len);
-        }, n_list, n_gaps);
-    }
+    wave_items(B.ring_off, B.n_halo, B.n_items, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+        const uint2 run = B.runs[item];
+        if (run.y == 0) return;
+        const HaloRec &h = B.halos[hi];
+        const int32_t ring = h.irmin + (int32_t)k;
+        int32_t npr, kshift; int64_t first;
+        ring_info(B.nside, ring, npr, kshift, first);
+        run_pieces((int32_t)run.x, (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
+                   [&](int32_t tile, int32_t lo, int32_t len) {
+            const uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
+            B.segs[B.tile_off[tile] + slot] = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
+        });
+    });
 }
 
-static unsigned bin_grid(int64_t n) { int64_t g = (n + 255) / 256; return (unsigned)(g > 8192 ? 8192 : g); }
+static unsigned bin_grid(int64_t n) { int64_t g = (n + 255) / 256; return (unsigned)(g > 8192 ? 8192 : (g < 1 ? 1 : g)); }
+void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ring_counts, dim3(bin_grid(b.n_halo + 1)), dim3(256), 0, s, b);
+}
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s)
 {
-    if (b.n_halo <= 0) return;
-    hipLaunchKernelGGL(k_seg_count, dim3(bin_grid(b.n_halo)), dim3(256), 0, s, b);
+    if (b.n_items == 0) return;
+    hipLaunchKernelGGL(k_seg_count, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_ring_gaps, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
 }
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s)
 {
-    if (b.n_halo <= 0) return;
-    hipLaunchKernelGGL(k_seg_emit, dim3(bin_grid(b.n_halo)), dim3(256), 0, s, b);
+    if (b.n_items == 0) return;
+    hipLaunchKernelGGL(k_seg_emit, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
 }
 
-/* exclusive scan of tile counts (uint32) into offsets (uint64) */
-int p2m_scan_tiles(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s)
+/* exclusive scan of uint32 counts into uint64 offsets with a 64-bit accumulator */
+int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s)
 {
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, cnt, off, (int)n, s);
+    hipError_t e = hipcub::DeviceScan::ExclusiveScan(tmp, *tmp_bytes, cnt, off, hipcub::Sum(), 0ULL, (int)n, s);
     return e == hipSuccess ? 0 : -1;
 }
+
 
 /* ===================================================================== */
 /* tiled painter                                                          */
@@ -693,12 +772,338 @@ __global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
     block_count(&A.counters[5], n_fb, lds16);
 }
 
+/* --------------------------------------------------------------------- */
+/* FP32 tiled painter                                                      */
+/*                                                                         */
+/* Per pixel: one FP64 FMA for the azimuth offset from the halo centre,    */
+/* then FP32: chord^2 = A + B sin^2(dphi/2) with the per-(halo,ring)       */
+/* constants A = (z-z0)^2 + (sth-s0)^2, B = 4 sth s0 staged in LDS;        */
+/* tan^2(theta) = 4h(1-h)/(1-2h)^2 with h = chord^2/4; the cell coordinate */
+/* u = (ln r - ln rmin)/drt from log2(r^2); fr = ln(r/r_node) from         */
+/* log1p(r^2/r_node^2 - 1).  Pixels whose chord^2 lies within band_c of    */
+/* the disc edge, or whose u lies within `band` of a cell boundary, the    */
+/* r cut-off or the r clamp, are deferred to k_fallback, which decides     */
+/* them with the exact FP64 arithmetic of the reference.                   */
+/* --------------------------------------------------------------------- */
+
+#define FSEG 256
+#define FB_LDS 512
+
+struct FastSeg {
+    uint32_t halo[FSEG];
+    uint32_t lo[FSEG];
+    uint32_t pre[FSEG + 1];
+    float A[FSEG], B[FSEG], thr_hi[FSEG], thr_lo[FSEG], chi2[FSEG], vh[FSEG];
+    float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];
+    int32_t colbase[FSEG];
+    double dphi0[FSEG];
+    unsigned long long hh[FSEG];
+    uint4 fb[FB_LDS];
+    uint32_t nfb;
+};
+
+/* sin^2(x), |x| <= pi/2, FP32 Taylor polynomials (to x^11 / x^10) */
+__device__ static inline float sin2f(float x)
+{
+    float ax = __builtin_fabsf(x);
+    if (ax <= 0.78539816f) {
+        float z = x * x;
+        float p = __builtin_fmaf(z, -2.5052108e-8f, 2.7557319e-6f);
+        p = __builtin_fmaf(z, p, -1.9841270e-4f);
+        p = __builtin_fmaf(z, p, 8.3333333e-3f);
+        p = __builtin_fmaf(z, p, -1.6666667e-1f);
+        float sn = __builtin_fmaf(x * z, p, x);
+        return sn * sn;
+    }
+    float y = 1.5707963f - ax;
+    float z = y * y;
+    float p = __builtin_fmaf(z, 2.0876757e-9f, -2.7557319e-7f);
+    p = __builtin_fmaf(z, p, 2.4801587e-5f);
+    p = __builtin_fmaf(z, p, -1.3888889e-3f);
+    p = __builtin_fmaf(z, p, 4.1666667e-2f);
+    p = __builtin_fmaf(z, p, -0.5f);
+    float cs = __builtin_fmaf(z, p, 1.0f);
+    return cs * cs;
+}
+
+/* ln(1+q) for 0 <= q < 0.08 (2 atanh(q/(2+q))) with the hardware reciprocal */
+__device__ static inline float log1p_small_rcp(float q)
+{
+    float s = q * __builtin_amdgcn_rcpf(2.0f + q);
+    float s2 = s * s;
+    float p = __builtin_fmaf(s2, 1.0f / 7.0f, 1.0f / 5.0f);
+    p = __builtin_fmaf(s2, p, 1.0f / 3.0f);
+    return 2.0f * __builtin_fmaf(s * s2, p, s);
+}
+
+template <int NM>
+__global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int W = 1 << A.log2w;
+    float *accS = (float *)smem_raw;
+    float *accC = accS + NM * W;
+    const size_t acc_bytes = (size_t)NM * W * 8;
+    uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes);
+    unsigned long long *lhash = (unsigned long long *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 4 : 0));
+    const size_t mem_bytes = A.count ? (size_t)W * 12 : 0;
+    FastSeg *sl = (FastSeg *)(smem_raw + acc_bytes + mem_bytes);
+    __shared__ uint32_t wsum[16];
+    __shared__ unsigned long long lds16[16];
+    __shared__ unsigned long long fb_base;
+
+    const int nrt = A.g.nrt;
+    const int64_t dmh = nrt, dch = (int64_t)nrt * A.g.nmht;
+    unsigned long long n_upd = 0, n_fb = 0;
+    if (threadIdx.x == 0) sl->nfb = 0;
+    __syncthreads();
+
+    for (int64_t t = blockIdx.x; t < A.n_tiles; t += gridDim.x) {
+        int lo_r = 0, hi_r = A.n_rings - 1;
+        while (lo_r < hi_r) {
+            int mid = (lo_r + hi_r + 1) >> 1;
+            if (A.tile_base[mid] <= t) lo_r = mid; else hi_r = mid - 1;
+        }
+        const int32_t ring = lo_r + 1;
+        const int32_t col = (int32_t)(t - A.tile_base[lo_r]);
+        int32_t npr, kshift; int64_t first;
+        ring_info(A.nside, ring, npr, kshift, first);
+        const int32_t ip0 = col * W;
+        const int32_t width = npr - ip0 < W ? npr - ip0 : W;
+        double zr, sth, off, den;
+        ring_zsth(A.nside, ring, zr, sth);
+        ring_phi_param(A.nside, ring, off, den);
+        const double kphi = HPX_HALFPI / den;
+
+        for (int q = threadIdx.x; q < width; q += blockDim.x) {
+            for (int m = 0; m < NM; ++m) { accS[m * W + q] = 0.0f; accC[m * W + q] = 0.0f; }
+            if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
+        }
+
+        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
+        for (uint64_t b0 = sbeg; b0 < send;) {
+            __syncthreads();
+            int nb = (int)(send - b0 < FSEG ? send - b0 : FSEG);
+            const int cap = (int)blockDim.x < FSEG ? (int)blockDim.x : FSEG;
+            if (nb > cap) nb = cap;
+            uint32_t len = 0;
+            const int tid = threadIdx.x;
+            if (tid < nb) {
+                uint2 sg = A.segs[b0 + tid];
+                const HaloRec &h = A.halos[sg.x];
+                sl->halo[tid] = sg.x;
+                sl->lo[tid] = sg.y >> 16;
+                len = sg.y & 0xffffu;
+                double dz = zr - h.zz0, ds = sth - h.s0;
+                sl->A[tid] = (float)(dz * dz + ds * ds);
+                sl->B[tid] = (float)((4.0 * sth) * h.s0);
+                sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
+                sl->thr_lo[tid] = (float)(h.thr * (1.0 - (double)A.band_c));
+                sl->chi2[tid] = (float)(h.chi_r * h.chi_r);
+                sl->vh[tid] = h.chi_out ? 0.0f : (float)h.vh;
+                float fc = (float)h.fc, fm = (float)h.fm;
+                sl->w00[tid] = (1.0f - fc) * (1.0f - fm);
+                sl->w10[tid] = fc * (1.0f - fm);
+                sl->w01[tid] = (1.0f - fc) * fm;
+                sl->w11[tid] = fc * fm;
+                sl->colbase[tid] = nrt * ((h.imh - 1) + A.g.nmht * (h.ichi - 1));
+                sl->dphi0[tid] = off * kphi - h.phi0;
+                sl->hh[tid] = halo_hash(h.cat);
+            }
+            uint32_t v = len;
+            const int lane = __lane_id();
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t u = __shfl_up(v, o);
+                if (lane >= o) v += u;
+            }
+            const int wv = threadIdx.x >> 6;
+            if (lane == 63) wsum[wv] = v;
+            __syncthreads();
+            uint32_t add = 0;
+            for (int k = 0; k < wv; ++k) add += wsum[k];
+            if (tid < nb) sl->pre[tid + 1] = v + add;
+            if (tid == 0) sl->pre[0] = 0;
+            __syncthreads();
+            const uint32_t P = sl->pre[nb];
+            b0 += (uint64_t)nb;
+            for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+                int s_lo = 0, s_hi = nb - 1;
+                while (s_lo < s_hi) {
+                    int mid = (s_lo + s_hi + 1) >> 1;
+                    if (sl->pre[mid] <= p) s_lo = mid; else s_hi = mid - 1;
+                }
+                const int s = s_lo;
+                const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
+                const int ip = ip0 + q;
+                double x = 0.5 * __builtin_fma((double)ip, kphi, sl->dphi0[s]);
+                if (x > HPX_HALFPI) x -= HPX_PI;
+                const float s2 = sin2f((float)x);
+                const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
+                if (c2 >= sl->thr_hi[s]) continue;
+                bool exact = c2 > sl->thr_lo[s];
+                float val[NM];
+                if (!exact) {
+                    const float h2 = 0.25f * c2;
+                    const float dd = 1.0f - 2.0f * h2;
+                    if (!(dd > 1e-3f)) {
+                        exact = true;
+                    } else {
+                        const float rd = __builtin_amdgcn_rcpf(dd);
+                        const float t2 = ((4.0f * h2) * (1.0f - h2)) * (rd * rd);
+                        const float r2 = sl->chi2[s] * t2;
+                        const float u = __builtin_amdgcn_logf(r2) * A.u_k1 - A.u_k0;
+                        const float fl = __builtin_floorf(u);
+                        const float fu = u - fl;
+                        if (u > A.u_max + A.band) {
+                            for (int m = 0; m < NM; ++m) val[m] = 0.0f;     /* r > rmaxt */
+                        } else if ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
+                                   (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))) {
+                            exact = true;
+                        } else {
+                            int ir;
+                            float fr;
+                            if (u < 0.0f) { ir = 1; fr = A.fr_clamp; }     /* r = rmint + 1e-5 */
+                            else {
+                                ir = (int)fl + 1;
+                                fr = 0.5f * log1p_small_rcp(__builtin_fmaf(r2, A.inv_node2f[ir - 1], -1.0f));
+                            }
+                            const int64_t c0 = (int64_t)sl->colbase[s] + (ir - 1);
+                            const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
+                            const float vh = sl->vh[s];
+                            for (int m = 0; m < NM; ++m) {
+                                const float *T = A.tab_prof[m];
+                                const float a0 = T[c0], a1 = T[c0 + 1];
+                                const float b0v = T[c0 + dmh], b1 = T[c0 + dmh + 1];
+                                const float cc0 = T[c0 + dch], cc1 = T[c0 + dch + 1];
+                                const float d0 = T[c0 + dmh + dch], d1 = T[c0 + dmh + dch + 1];
+                                float acc = w00 * __builtin_fmaf(fr, a1 - a0, a0);
+                                acc = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc);
+                                acc = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc);
+                                acc = __builtin_fmaf(w11, __builtin_fmaf(fr, d1 - d0, d0), acc);
+                                val[m] = vh * acc;
+                            }
+                        }
+                    }
+                }
+                if (exact) {
+                    uint32_t slot = atomicAdd(&sl->nfb, 1u);
+                    uint4 e = make_uint4((uint32_t)ring, (uint32_t)ip, sl->halo[s], 0u);
+                    if (slot < FB_LDS) sl->fb[slot] = e;
+                    else {
+                        unsigned long long gs = atomicAdd(A.fb_count, 1ULL);
+                        if (gs < (unsigned long long)A.fb_cap) A.fb[gs] = e;
+                    }
+                    n_fb++;
+                    continue;
+                }
+                n_upd++;
+                for (int m = 0; m < NM; ++m) {
+                    const float vv = val[m];
+                    const float old = atomicAdd(&accS[m * W + q], vv);
+                    const float sum = old + vv;
+                    const float bb = sum - old;
+                    const float err = (old - (sum - bb)) + (vv - bb);
+                    atomicAdd(&accC[m * W + q], err);
+                }
+                if (A.count) {
+                    atomicAdd(&lcount[q], 1u);
+                    atomicAdd(&lhash[q], sl->hh[s]);
+                }
+            }
+        }
+        __syncthreads();
+        const int64_t pix0 = first + ip0;
+        for (int q = threadIdx.x; q < width; q += blockDim.x) {
+            for (int m = 0; m < NM; ++m) {
+                double v = (double)accS[m * W + q] + (double)accC[m * W + q];
+                float *dst = A.map32[m] + pix0 + q;
+                *dst = A.accumulate ? (float)((double)*dst + v) : (float)v;
+            }
+            if (A.count) {
+                uint32_t *dc = A.count + pix0 + q;
+                unsigned long long *dh = A.hash + pix0 + q;
+                *dc = A.accumulate ? *dc + lcount[q] : lcount[q];
+                *dh = A.accumulate ? *dh + lhash[q] : lhash[q];
+            }
+        }
+        /* flush the deferred-pixel buffer when half full */
+        if (sl->nfb >= FB_LDS / 2) {
+            uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
+            if (threadIdx.x == 0) fb_base = atomicAdd(A.fb_count, (unsigned long long)nf);
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x)
+                if (fb_base + k < (unsigned long long)A.fb_cap) A.fb[fb_base + k] = sl->fb[k];
+            __syncthreads();
+            if (threadIdx.x == 0) sl->nfb = 0;
+        }
+        __syncthreads();
+    }
+    {
+        uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
+        if (nf) {
+            if (threadIdx.x == 0) fb_base = atomicAdd(A.fb_count, (unsigned long long)nf);
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x)
+                if (fb_base + k < (unsigned long long)A.fb_cap) A.fb[fb_base + k] = sl->fb[k];
+        }
+    }
+    block_count(&A.counters[1], n_upd, lds16);
+    block_count(&A.counters[5], n_fb, lds16);
+}
+
+/* Deferred pixels of k_tile_fast, decided and valued with the exact FP64
+ * arithmetic of the reference (pix2vec_ring, angdist, interpolate_table),
+ * added to the FP32 map with one atomic each. */
+__global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
+{
+    __shared__ unsigned long long lds16[16];
+    unsigned long long n = *A.fb_count;
+    if (n > (unsigned long long)A.fb_cap) n = (unsigned long long)A.fb_cap;
+    unsigned long long n_upd = 0;
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint4 e = A.fb[i];
+        const int32_t ring = (int32_t)e.x, ip = (int32_t)e.y;
+        const HaloRec &h = A.halos[e.z];
+        int32_t npr, kshift; int64_t first;
+        ring_info(A.nside, ring, npr, kshift, first);
+        double zr, sth, off, den;
+        ring_zsth(A.nside, ring, zr, sth);
+        ring_phi_param(A.nside, ring, off, den);
+        const double phi = (((double)ip + off) * HPX_HALFPI) / den;
+        double sn, cs;
+        p2m_sincos(phi, &sn, &cs);
+        const double angle = angdist(h.hx, h.hy, h.hz, sth * cs, sth * sn, zr);
+        if (!(angle < h.dtheta)) continue;
+        n_upd++;
+        const int64_t pix = first + ip;
+        for (int m = 0; m < A.n_maps; ++m) {
+            double v = h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle);
+            atomicAdd(A.map32[m] + pix, (float)v);
+        }
+        if (A.count) {
+            atomicAdd(&A.count[pix], 1u);
+            atomicAdd(&A.hash[pix], (unsigned long long)halo_hash(h.cat));
+        }
+    }
+    block_count(&A.counters[1], n_upd, lds16);
+}
+
+int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)
+{
+    int64_t g = (n_max + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_fallback, dim3((unsigned)g), dim3(256), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership)
 {
     size_t W = (size_t)1 << log2w;
     size_t b = (size_t)n_maps * W * 8;
     if (membership) b += W * 12;
-    b += sizeof(SegLDS);
+    b += mode == 2 ? sizeof(FastSeg) : sizeof(SegLDS);
     return b;
 }
 
@@ -713,12 +1118,15 @@ int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int
     size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr);
     if (lds > 160 * 1024) return -1;
 #define P2M_TL(MD, NMv) case NMv: launch_tile<MD, NMv>(a, grid, wg, lds, s); break;
+#define P2M_TF(NMv) case NMv: hipLaunchKernelGGL((k_tile_fast<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
     if (mode == 1) {
         switch (n_maps) { P2M_TL(1, 1) P2M_TL(1, 2) P2M_TL(1, 3) P2M_TL(1, 4) P2M_TL(1, 5) P2M_TL(1, 6) default: return -1; }
     } else {
-        switch (n_maps) { P2M_TL(2, 1) P2M_TL(2, 2) P2M_TL(2, 3) P2M_TL(2, 4) P2M_TL(2, 5) P2M_TL(2, 6) default: return -1; }
+        if (wg > 256) return -1;
+        switch (n_maps) { P2M_TF(1) P2M_TF(2) P2M_TF(3) P2M_TF(4) P2M_TF(5) P2M_TF(6) default: return -1; }
     }
 #undef P2M_TL
+#undef P2M_TF
     return 0;
 }
 
