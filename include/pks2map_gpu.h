@@ -55,7 +55,7 @@ typedef struct {
     int32_t profile;      /* 1 tSZ (v = 1), 2 kSZ (v = -v_r/c) (pksc.f90:287-295) */
     int32_t n_maps;       /* tables painted in one catalogue pass (1..6) */
     int32_t precision;    /* P2M_PREC_* */
-    int64_t chunk_halos;  /* halos per streamed chunk (0: 1<<24) */
+    int64_t chunk_halos;  /* halos per streamed chunk (0: 1<<24; at most 1<<30) */
     int32_t device;       /* HIP device ordinal */
     int32_t flags;        /* P2M_FLAG_* */
 } p2m_config;
@@ -82,6 +82,7 @@ typedef struct {
     double  cut_low, cut_high;
     double  ms_total, ms_decode, ms_bin, ms_paint, ms_finish;  /* last p2m_paint* call */
     int32_t tile_w, tile_wg;  /* tile shape used (autotuned or set) */
+    double  ms_tile;          /* the tile painter kernel alone, within ms_paint (0 for plain FP64) */
 } p2m_stats;
 
 /* Context on one GPU; allocates the maps (zeroed).  Replaces the allocations

@@ -44,7 +44,8 @@ class Stats(C.Structure):
                 ("n_segments", C.c_int64), ("n_ring_gaps", C.c_int64), ("n_radius_err", C.c_int64),
                 ("cut_low", C.c_double), ("cut_high", C.c_double), ("ms_total", C.c_double),
                 ("ms_decode", C.c_double), ("ms_bin", C.c_double), ("ms_paint", C.c_double),
-                ("ms_finish", C.c_double), ("tile_w", C.c_int32), ("tile_wg", C.c_int32)]
+                ("ms_finish", C.c_double), ("tile_w", C.c_int32), ("tile_wg", C.c_int32),
+                ("ms_tile", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

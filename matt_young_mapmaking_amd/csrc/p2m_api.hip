@@ -43,7 +43,7 @@ namespace {
 const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
 
 struct Ctr {                 /* layout of the device counter block */
-    enum { N_OUT = 0, DEC = 1, N_FLAT = 5, PAINT = 8, TOTAL = 16 };
+    enum { N_OUT = 0, DEC = 1, N_FLAT = 5, N_CAND = 6, PAINT = 8, TOTAL = 16 };
 };
 
 }  // namespace
@@ -100,6 +100,8 @@ struct p2m_ctx {
     int64_t seg_cap = 0;
     float *d_stage = nullptr;
     int64_t stage_cap = 0;
+    uint32_t *d_cand = nullptr;
+    int64_t cand_cap = 0;
     p2m_stats st;
 };
 
@@ -201,6 +203,7 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
     p2m_ctx *c = new p2m_ctx();
     c->cfg = *cfg;
     if (c->cfg.chunk_halos <= 0) c->cfg.chunk_halos = 1 << 24;
+    if (c->cfg.chunk_halos > (1LL << 30)) c->cfg.chunk_halos = 1LL << 30;   /* chunk indices are 32-bit */
     memset(&c->st, 0, sizeof(c->st));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return P2M_E_HIP; }
     for (int i = 0; i < 8; ++i) (void)hipEventCreate(&c->ev[i]);
@@ -244,7 +247,7 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
-    dfree(c->d_segs); dfree(c->d_stage);
+    dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
     for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -434,9 +437,11 @@ static int scan_counts(p2m_ctx *c, const uint32_t *cnt, unsigned long long *off,
     return 0;
 }
 
+/* The per-halo buffers exist after the first paint call even for an empty
+ * catalogue: the binning writes the terminating ring count nr[n_halo]. */
 static int ensure_halo_cap(p2m_ctx *c, int64_t n)
 {
-    if (n <= c->halo_cap) return 0;
+    if (n <= c->halo_cap && c->d_halo) return 0;
     dfree(c->d_halo); c->d_halo = nullptr; c->halo_cap = 0;
     dfree(c->d_flat_idx); c->d_flat_idx = nullptr;
     dfree(c->d_nr); c->d_nr = nullptr;
@@ -464,6 +469,8 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
     a.out = c->d_halo; a.cap = c->halo_cap;
     a.n_out = c->d_ctr + Ctr::N_OUT;
     a.flat_idx = c->d_flat_idx;
+    a.cand = c->d_cand;
+    a.n_cand = c->d_ctr + Ctr::N_CAND;
     a.n_flat_out = c->d_ctr + Ctr::N_FLAT;
     a.counters = c->d_ctr + Ctr::DEC;
     p2m_launch_decode(a, c->stream);
@@ -517,7 +524,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         unsigned long long n_items = 0;
         HIPCHK(hipMemcpyAsync(&n_items, c->d_ring_off + n_halo, sizeof(n_items), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        if ((int64_t)n_items > c->runs_cap) {
+        if ((int64_t)n_items > c->runs_cap || !c->d_runs) {
             dfree(c->d_runs); c->d_runs = nullptr; c->runs_cap = 0;
             int64_t cap = (int64_t)(n_items * 1.25) + 1024;
             if (dmalloc((void **)&c->d_runs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
@@ -541,7 +548,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
                 c->fb_cap = need;
             }
         }
-        if ((int64_t)nseg > c->seg_cap) {
+        if ((int64_t)nseg > c->seg_cap || !c->d_segs) {
             dfree(c->d_segs); c->d_segs = nullptr; c->seg_cap = 0;
             int64_t cap = (int64_t)(nseg * 1.25) + 1024;
             if (dmalloc((void **)&c->d_segs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
@@ -569,6 +576,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         int grid = (int)std::min<int64_t>(c->n_tiles, 8192);
         if (p2m_launch_tile_paint(t, mode, c->cfg.n_maps, grid, c->wg, c->stream)) return P2M_E_INVALID;
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[7], c->stream));
         if (mode == P2M_PREC_FP32_COMP) {
             FallbackArgs f;
             memset(&f, 0, sizeof(f));
@@ -618,6 +626,10 @@ static int finish_stats(p2m_ctx *c)
     (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]); s.ms_bin = ms;
     (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); s.ms_paint = ms;
     (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]); s.ms_finish = ms;
+    s.ms_tile = 0.0;
+    if (c->cfg.precision != P2M_PREC_PLAIN_FP64 && c->cfg.nside > 0) {
+        (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[7]); s.ms_tile = ms;
+    }
     s.tile_w = c->log2w ? (1 << c->log2w) : 0;
     s.tile_wg = c->wg;
     return 0;
@@ -634,6 +646,11 @@ static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n,
     p2m_launch_fill_u64(c->d_ctr, Ctr::TOTAL, c->stream);
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     const int64_t chunk = c->cfg.chunk_halos;
+    if (n > 0 && std::min(chunk, n) > c->cand_cap) {
+        dfree(c->d_cand); c->d_cand = nullptr; c->cand_cap = 0;
+        if (dmalloc((void **)&c->d_cand, sizeof(uint32_t) * std::min(chunk, n))) return P2M_E_NOMEM;
+        c->cand_cap = std::min(chunk, n);
+    }
     if (!on_device && n > 0) {
         int64_t need = std::min(chunk, n) * 7;
         if (need > c->stage_cap) {

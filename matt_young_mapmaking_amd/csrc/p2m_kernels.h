@@ -18,6 +18,8 @@ struct DecodeArgs {
     int32_t precut;
     int32_t profile, nside, npix_fs;
     double fov, dpix;
+    uint32_t *cand;              /* chunk indices that passed both cuts */
+    unsigned long long *n_cand;
     HaloRec *out;
     int64_t cap;
     unsigned long long *n_out;   /* append counter */

@@ -1,7 +1,8 @@
 /*
  * p2m_kernels.hip -- gfx950 kernels of the halo->map painter.
  *
- *  k_decode          catalogue decode + cull + FP64 per-halo setup
+ *  k_cull, k_setup   catalogue decode: radial and mass cuts over every record,
+ *                    then the FP64 per-halo setup of the survivors
  *                    (pks2map.f90:144-160, 264, 284-298; haloproject.f90:47-55,
  *                    93-128; maptable.f90:184-202; query_disc prologue)
  *  k_paint_plain_hp  baseline: one work item per halo, naive query_disc loop,
@@ -87,30 +88,86 @@ __device__ static inline void block_count(unsigned long long *counter, unsigned 
 /* decode                                                                 */
 /* ===================================================================== */
 
-__global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
+/* workgroup-level append: returns this thread's slot (valid if take);
+ * one atomic per workgroup; every thread calls */
+__device__ static inline unsigned long long block_append(unsigned long long *counter, bool take,
+                                                         unsigned int *wcnt, unsigned long long *base_s)
+{
+    const unsigned long long mask = __ballot(take);
+    const int lane = __lane_id(), wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    if (lane == 0) wcnt[wv] = (unsigned int)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+        for (int k = 0; k < nwv; ++k) t += wcnt[k];
+        *base_s = t ? atomicAdd(counter, (unsigned long long)t) : 0ULL;
+    }
+    __syncthreads();
+    unsigned long long off = *base_s;
+    for (int k = 0; k < wv; ++k) off += wcnt[k];
+    const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+    off += (unsigned long long)__popcll(below);
+    __syncthreads();
+    return off;
+}
+
+/* Cull pass over every record: the radial cut (pks2map.f90:144-160) and the
+ * mass cut on the unconverted mass (:264, :296).  Survivors' chunk indices
+ * are appended to `cand`; the FP64 setup then runs on full waves. */
+__global__ void __launch_bounds__(256) k_cull(DecodeArgs A)
+{
+    const CosmoDev &c = A.c;
+    __shared__ unsigned long long lds16[16];
+    __shared__ unsigned int wcnt[16];
+    __shared__ unsigned long long base_s;
+    unsigned long long tot_cut = 0;
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < A.n; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        bool take = false;
+        if (i < A.n) {
+            const float *p = A.halos + 7 * i;
+            const double xh = (double)p[2], yh = (double)p[1], zh = (double)p[0];
+            const double xs = xh - A.chihview;
+            const double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
+            if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
+                tot_cut++;
+                const double mass = (((4.0 / 3.0) * c.mypi) * c.rho_0) * cube((double)p[6]);
+                take = !(mass < A.mmin);
+            }
+        }
+        const unsigned long long slot = block_append(A.n_cand, take, wcnt, &base_s);
+        if (take) A.cand[slot] = (uint32_t)i;
+    }
+    block_count(&A.counters[0], tot_cut, lds16);
+}
+
+/* FP64 per-halo setup of the survivors: redshift, mass conversion, disc
+ * radius (pks2map.f90:284-298, haloproject.f90:47), the query_disc and
+ * discphirange_at_z prologues (pixel_routines.F90:1981-2010,
+ * pix_tools.F90:729-738), the per-halo part of interpolate_table
+ * (maptable.f90:184-202) and the flat-sky box (haloproject.f90:101-128). */
+__global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
 {
     const CosmoDev &c = A.c;
     const TabGeom &g = A.g;
     __shared__ unsigned long long lds16[16];
-    __shared__ unsigned int wcnt[4];
+    __shared__ unsigned int wcnt[16];
     __shared__ unsigned long long base_s;
-    unsigned long long tot_cut = 0, tot_rad = 0, tot_painted = 0;
-    for (int64_t i0 = blockIdx.x * 256LL; i0 < A.n; i0 += (int64_t)gridDim.x * 256) {
-    unsigned long long n_cut = 0, n_rad = 0;
-    int64_t i = i0 + threadIdx.x;
-    bool painted = false;
-    HaloRec h;
-    if (i < A.n) {
-        const float *p = A.halos + 7 * i;
-        float p0 = p[0], p1 = p[1], p2 = p[2], p6 = p[6];
-        /* radial cut (pks2map.f90:144-160), painter axes x=pos3, y=pos2, z=pos1 */
-        double xh = (double)p2, yh = (double)p1, zh = (double)p0;
-        double xs = xh - A.chihview;
-        double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
-        if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
-            n_cut = 1;
-            double rth = (double)p6;
-            double mass = (((4.0 / 3.0) * c.mypi) * c.rho_0) * cube(rth);   /* pks2map.f90:264 */
+    unsigned long long tot_rad = 0, tot_painted = 0;
+    const unsigned long long ncand = *A.n_cand;
+    for (unsigned long long k0 = blockIdx.x * (unsigned long long)blockDim.x; k0 < ncand;
+         k0 += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long k = k0 + threadIdx.x;
+        bool painted = false;
+        HaloRec h;
+        if (k < ncand) {
+            const int64_t i = A.cand[k];
+            const float *p = A.halos + 7 * i;
+            const float p0 = p[0], p1 = p[1], p2 = p[2], p6 = p[6];
+            const double xh = (double)p2, yh = (double)p1, zh = (double)p0;
+            const double xs = xh - A.chihview;
+            const double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
+            const double mass = (((4.0 / 3.0) * c.mypi) * c.rho_0) * cube((double)p6);   /* pks2map.f90:264 */
             double vh = 1.0;
             if (A.profile == 2) {
                 double px = p0, py = p1, pz = p2;
@@ -118,112 +175,88 @@ __global__ void __launch_bounds__(256) k_decode(DecodeArgs A)
                 vr = vr / p2m_sqrt((px * px + py * py) + pz * pz);
                 vh = -vr / 3.0e5;
             }
-            double chi_pre = p2m_sqrt((xh * xh + yh * yh) + zh * zh);          /* :289 */
-            double redshift = tab_lookup(c.zofr, c.nzofr, c.zofr_min, c.dzofr, chi_pre);  /* :290 */
-            double mh = p2m_sqrt(deltacrit_z(c, redshift) / A.dmodel) * mass;    /* :295 */
-            if (!(mass < A.mmin)) {                                            /* :296 */
-                double rh = p2m_pow(((((3.0 * mh) / 4.0) / c.mypi) / 200.0) / rhocrit_z(c, redshift), 1.0 / 3.0); /* :298 */
-                double dtheta = 4.0 * p2m_asin(rh / chih);                      /* haloproject.f90:47 */
-                if (!(dtheta >= 0.0 && dtheta <= HPX_PI)) {
-                    n_rad = 1;   /* pixel_routines.F90:1962 would abort */
-                } else {
-                    painted = true;
-                    double ux = xs / chih, uy = yh / chih, uz = zh / chih;
-                    h.hx = uz; h.hy = uy; h.hz = ux;
-                    h.dtheta = dtheta;
-                    h.vh = vh;
-                    h.cat = A.index_base + i;
-                    /* query_disc prologue (pixel_routines.F90:1981-2010) */
-                    double norm = p2m_sqrt((h.hx * h.hx + h.hy * h.hy) + h.hz * h.hz);
-                    double z0 = h.hz / norm;
-                    h.irmin = 0; h.irmax = -1;
-                    if (A.nside > 0) {
-                        double rlat0 = p2m_asin(z0);
-                        double rlat1 = rlat0 + dtheta, rlat2 = rlat0 - dtheta;
-                        double zmax = (rlat1 >= HPX_HALFPI) ? 1.0 : p2m_sin(rlat1);
-                        int32_t irmin = ring_num(A.nside, zmax);
-                        h.irmin = irmin - 1 > 1 ? irmin - 1 : 1;
-                        double zmin = (rlat2 <= -HPX_HALFPI) ? -1.0 : p2m_sin(rlat2);
-                        int32_t irmax = ring_num(A.nside, zmin);
-                        h.irmax = irmax + 1 < 4 * A.nside - 1 ? irmax + 1 : 4 * A.nside - 1;
-                    }
-                    /* discphirange_at_z prologue (pix_tools.F90:729-738) */
-                    h.cosang = p2m_cos(dtheta);
-                    double x0 = h.hx / norm, y0 = h.hy / norm;
-                    h.zz0 = h.hz / norm;
-                    h.phi0 = 0.0;
-                    if (x0 != 0.0 || y0 != 0.0) h.phi0 = p2m_atan2(y0, x0);
-                    h.a = x0 * x0 + y0 * y0;
-                    h.s0 = p2m_sqrt(h.a);
-                    double sh = p2m_sin(0.5 * dtheta);
-                    h.thr = 4.0 * (sh * sh);
-                    /* interpolate_table per-halo part (maptable.f90:184-202) */
-                    double chi = tab_lookup(c.rofz, c.nrofz, c.rofz_min, c.drofz, redshift);
-                    h.chi_r = chi;
-                    h.chi_out = chi > g.chimaxt ? 1 : 0;
-                    if (chi < g.chimint) chi = g.chimint + 1e-5;
-                    double mc = mh;
-                    if (mc < g.mhmint) mc = g.mhmint + 1e-5;
-                    if (mc > g.mhmaxt) mc = g.mhmaxt - 1e-5;
-                    double lc = p2m_log(chi), lm = p2m_log(mc);
-                    h.ichi = (int)((lc - g.lc0) / g.dchit) + 1;
-                    h.imh = (int)((lm - g.lm0) / g.dmht) + 1;
-                    h.fc = lc - (g.lc0 + (double)(h.ichi - 1) * g.dchit);
-                    h.fm = lm - (g.lm0 + (double)(h.imh - 1) * g.dmht);
-                    /* flat-sky bounding box (haloproject.f90:101-128) */
-                    h.flat = 0; h.th = 0.0; h.ph = 0.0;
-                    h.jmin = h.jmax = h.kmin = h.kmax = 0;
-                    if (A.npix_fs > 0 && !(uz < 0.0)) {
-                        double th = p2m_asin(ux), ph = p2m_asin(uy);
-                        h.th = th; h.ph = ph;
-                        double thetamin = th - 1.1 * dtheta, thetamax = th + 1.1 * dtheta;
-                        double phimin = ph - 1.1 * dtheta, phimax = ph + 1.1 * dtheta;
-                        double hf = A.fov / 2.0;
-                        if (!(thetamax < -hf || thetamin > hf) && !(phimax < -hf || phimin > hf)) {
-                            if (thetamin <= -hf) thetamin = -hf + 1e-10;
-                            if (thetamax >= hf) thetamax = hf - 1e-10;
-                            if (phimin <= -hf) phimin = -hf + 1e-10;
-                            if (phimax >= hf) phimax = hf - 1e-10;
-                            int jmin = (int)((thetamin + hf) / A.dpix) + 1; if (jmin < 1) jmin = 1;
-                            int jmax = (int)((thetamax + hf) / A.dpix) + 1; if (jmax > A.npix_fs) jmax = A.npix_fs;
-                            int kmin = (int)((phimin + hf) / A.dpix) + 1; if (kmin < 1) kmin = 1;
-                            int kmax = (int)((phimax + hf) / A.dpix) + 1; if (kmax > A.npix_fs) kmax = A.npix_fs;
-                            if (jmax >= jmin && kmax >= kmin) {
-                                h.flat = 1; h.jmin = jmin; h.jmax = jmax; h.kmin = kmin; h.kmax = kmax;
-                            }
+            const double chi_pre = p2m_sqrt((xh * xh + yh * yh) + zh * zh);          /* :289 */
+            const double redshift = tab_lookup(c.zofr, c.nzofr, c.zofr_min, c.dzofr, chi_pre);  /* :290 */
+            const double mh = p2m_sqrt(deltacrit_z(c, redshift) / A.dmodel) * mass;    /* :295 */
+            const double rh = p2m_pow(((((3.0 * mh) / 4.0) / c.mypi) / 200.0) / rhocrit_z(c, redshift), 1.0 / 3.0); /* :298 */
+            const double dtheta = 4.0 * p2m_asin(rh / chih);                      /* haloproject.f90:47 */
+            if (!(dtheta >= 0.0 && dtheta <= HPX_PI)) {
+                tot_rad++;   /* pixel_routines.F90:1962 would abort */
+            } else {
+                painted = true;
+                const double ux = xs / chih, uy = yh / chih, uz = zh / chih;
+                h.hx = uz; h.hy = uy; h.hz = ux;
+                h.dtheta = dtheta;
+                h.vh = vh;
+                h.cat = A.index_base + i;
+                const double norm = p2m_sqrt((h.hx * h.hx + h.hy * h.hy) + h.hz * h.hz);
+                const double z0 = h.hz / norm;
+                h.irmin = 0; h.irmax = -1;
+                if (A.nside > 0) {
+                    const double rlat0 = p2m_asin(z0);
+                    const double rlat1 = rlat0 + dtheta, rlat2 = rlat0 - dtheta;
+                    const double zmax = (rlat1 >= HPX_HALFPI) ? 1.0 : p2m_sin(rlat1);
+                    const int32_t irmin = ring_num(A.nside, zmax);
+                    h.irmin = irmin - 1 > 1 ? irmin - 1 : 1;
+                    const double zmin = (rlat2 <= -HPX_HALFPI) ? -1.0 : p2m_sin(rlat2);
+                    const int32_t irmax = ring_num(A.nside, zmin);
+                    h.irmax = irmax + 1 < 4 * A.nside - 1 ? irmax + 1 : 4 * A.nside - 1;
+                }
+                h.cosang = p2m_cos(dtheta);
+                const double x0 = h.hx / norm, y0 = h.hy / norm;
+                h.zz0 = h.hz / norm;
+                h.phi0 = 0.0;
+                if (x0 != 0.0 || y0 != 0.0) h.phi0 = p2m_atan2(y0, x0);
+                h.a = x0 * x0 + y0 * y0;
+                h.s0 = p2m_sqrt(h.a);
+                const double sh = p2m_sin(0.5 * dtheta);
+                h.thr = 4.0 * (sh * sh);
+                double chi = tab_lookup(c.rofz, c.nrofz, c.rofz_min, c.drofz, redshift);
+                h.chi_r = chi;
+                h.chi_out = chi > g.chimaxt ? 1 : 0;
+                if (chi < g.chimint) chi = g.chimint + 1e-5;
+                double mc = mh;
+                if (mc < g.mhmint) mc = g.mhmint + 1e-5;
+                if (mc > g.mhmaxt) mc = g.mhmaxt - 1e-5;
+                const double lc = p2m_log(chi), lm = p2m_log(mc);
+                h.ichi = (int)((lc - g.lc0) / g.dchit) + 1;
+                h.imh = (int)((lm - g.lm0) / g.dmht) + 1;
+                h.fc = lc - (g.lc0 + (double)(h.ichi - 1) * g.dchit);
+                h.fm = lm - (g.lm0 + (double)(h.imh - 1) * g.dmht);
+                h.flat = 0; h.th = 0.0; h.ph = 0.0;
+                h.jmin = h.jmax = h.kmin = h.kmax = 0;
+                if (A.npix_fs > 0 && !(uz < 0.0)) {
+                    const double th = p2m_asin(ux), ph = p2m_asin(uy);
+                    h.th = th; h.ph = ph;
+                    double thetamin = th - 1.1 * dtheta, thetamax = th + 1.1 * dtheta;
+                    double phimin = ph - 1.1 * dtheta, phimax = ph + 1.1 * dtheta;
+                    const double hf = A.fov / 2.0;
+                    if (!(thetamax < -hf || thetamin > hf) && !(phimax < -hf || phimin > hf)) {
+                        if (thetamin <= -hf) thetamin = -hf + 1e-10;
+                        if (thetamax >= hf) thetamax = hf - 1e-10;
+                        if (phimin <= -hf) phimin = -hf + 1e-10;
+                        if (phimax >= hf) phimax = hf - 1e-10;
+                        int jmin = (int)((thetamin + hf) / A.dpix) + 1; if (jmin < 1) jmin = 1;
+                        int jmax = (int)((thetamax + hf) / A.dpix) + 1; if (jmax > A.npix_fs) jmax = A.npix_fs;
+                        int kmin = (int)((phimin + hf) / A.dpix) + 1; if (kmin < 1) kmin = 1;
+                        int kmax = (int)((phimax + hf) / A.dpix) + 1; if (kmax > A.npix_fs) kmax = A.npix_fs;
+                        if (jmax >= jmin && kmax >= kmin) {
+                            h.flat = 1; h.jmin = jmin; h.jmax = jmax; h.kmin = kmin; h.kmax = kmax;
                         }
                     }
                 }
             }
         }
-    }
-    /* block-level append: one atomic per 256 halos */
-    unsigned long long mask = __ballot(painted);
-    int lane = __lane_id(), wv = threadIdx.x >> 6;
-    if (lane == 0) wcnt[wv] = (unsigned int)__popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int t = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-        base_s = t ? atomicAdd(A.n_out, (unsigned long long)t) : 0ULL;
-    }
-    __syncthreads();
-    if (painted) {
-        unsigned long long off = base_s;
-        for (int k = 0; k < wv; ++k) off += wcnt[k];
-        unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-        off += (unsigned long long)__popcll(below);
-        if (off < (unsigned long long)A.cap) {
+        const unsigned long long off = block_append(A.n_out, painted, wcnt, &base_s);
+        if (painted && off < (unsigned long long)A.cap) {
             A.out[off] = h;
             if (h.flat) {
-                unsigned long long fs = atomicAdd(A.n_flat_out, 1ULL);
+                const unsigned long long fs = atomicAdd(A.n_flat_out, 1ULL);
                 A.flat_idx[fs] = (uint32_t)off;
             }
         }
+        tot_painted += painted ? 1ULL : 0ULL;
     }
-    tot_cut += n_cut; tot_rad += n_rad; tot_painted += painted ? 1ULL : 0ULL;
-    __syncthreads();
-    }
-    block_count(&A.counters[0], tot_cut, lds16);
     block_count(&A.counters[2], tot_rad, lds16);
     block_count(&A.counters[1], tot_painted, lds16);
 }
@@ -233,7 +266,9 @@ void p2m_launch_decode(const DecodeArgs &a, hipStream_t s)
     if (a.n <= 0) return;
     int64_t g = (a.n + 255) / 256;
     if (g > 8192) g = 8192;          /* grid-stride: bounded number of append atomics */
-    hipLaunchKernelGGL(k_decode, dim3((unsigned)g), dim3(256), 0, s, a);
+    p2m_launch_fill_u64(a.n_cand, 1, s);
+    hipLaunchKernelGGL(k_cull, dim3((unsigned)g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_setup, dim3((unsigned)g), dim3(256), 0, s, a);
 }
 
 /* ===================================================================== */

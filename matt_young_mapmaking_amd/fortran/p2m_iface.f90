@@ -39,6 +39,7 @@ module p2m_iface
      real(c_double)     :: cut_low, cut_high
      real(c_double)     :: ms_total, ms_decode, ms_bin, ms_paint, ms_finish
      integer(c_int32_t) :: tile_w, tile_wg
+     real(c_double)     :: ms_tile
   end type p2m_stats
 
   interface
