@@ -151,6 +151,13 @@ int p2m_make_maptable(int32_t device, int32_t model, double nu, double alpha_cmb
  * 3 tan, 4 asin, 5 acos, 6 atan2(x, y), 7 log, 8 exp, 9 pow(x, y). */
 int p2m_selftest_math(int32_t device, int32_t op, const double *x, const double *y, double *out, int64_t n);
 
+/* Measurement helper (no counterpart in the reference): copies n_bytes
+ * (a multiple of 16, at most 1<<34) between two device buffers `reps` times
+ * with a plain streaming kernel and returns the mean kernel time in *ms.
+ * Its known traffic (n_bytes read, n_bytes written per launch) calibrates
+ * the HBM byte counters of a profiler. */
+int p2m_selftest_copy(int32_t device, int64_t n_bytes, int32_t reps, double *ms);
+
 /* Library build identification. */
 const char *p2m_version(void);
 

@@ -24,7 +24,7 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "size mismatch", -3: "HIP error",
 EXPORTS = ["p2m_create", "p2m_destroy", "p2m_strerror", "p2m_set_run", "p2m_rofz", "p2m_zofr", "p2m_load_table",
            "p2m_paint", "p2m_paint_device", "p2m_clear", "p2m_fetch_maps", "p2m_fetch_membership", "p2m_get_stats",
            "p2m_set_tile", "p2m_products", "p2m_write_outputs", "p2m_make_maptable", "p2m_selftest_math",
-           "p2m_version"]
+           "p2m_selftest_copy", "p2m_version"]
 
 
 class Config(C.Structure):
@@ -86,6 +86,7 @@ def load():
     L.p2m_make_maptable.argtypes = [i32, i32, d, d, i32, i32, i32, P(i32), P(C.c_float), P(C.c_float), i64]
     L.p2m_version.restype = C.c_char_p
     L.p2m_selftest_math.argtypes = [i32, i32, P(d), P(d), P(d), i64]
+    L.p2m_selftest_copy.argtypes = [i32, i64, i32, P(d)]
     _lib = L
     return L
 
@@ -228,3 +229,12 @@ def selftest_math(op, x, y=None, device=0):
     _chk(L.p2m_selftest_math(device, MATH_OPS.index(op), _p(x, C.c_double), _p(yy, C.c_double),
                              _p(out, C.c_double), x.size), "p2m_selftest_math")
     return out
+
+
+def selftest_copy(n_bytes, reps=5, device=0):
+    """Mean time (ms) of the library's plain device-to-device copy kernel over
+    n_bytes (read n_bytes, write n_bytes per launch)."""
+    L = load()
+    ms = C.c_double(0.0)
+    _chk(L.p2m_selftest_copy(device, int(n_bytes), int(reps), C.byref(ms)), "p2m_selftest_copy")
+    return ms.value

@@ -1011,3 +1011,34 @@ extern "C" int p2m_selftest_math(int32_t device, int32_t op, const double *x, co
     (void)hipGetLastError();
     return rc;
 }
+
+extern "C" int p2m_selftest_copy(int32_t device, int64_t n_bytes, int32_t reps, double *ms)
+{
+    if (n_bytes <= 0 || n_bytes % 16 || n_bytes > (1LL << 34) || reps <= 0 || !ms) return P2M_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { (void)hipGetLastError(); return P2M_E_NODEVICE; }
+    if (device < 0 || device >= ndev) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = dmalloc(&a, (size_t)n_bytes);
+    if (!rc) rc = dmalloc(&b, (size_t)n_bytes);
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = P2M_E_HIP;
+    if (!rc) {
+        p2m_launch_fill_f32((float *)a, n_bytes / 4, 0);
+        if (p2m_launch_copy_calib(a, b, n_bytes / 16, 0)) rc = P2M_E_HIP;      /* warm-up */
+    }
+    if (!rc && hipEventRecord(e0, 0) != hipSuccess) rc = P2M_E_HIP;
+    for (int r = 0; r < reps && !rc; ++r)
+        if (p2m_launch_copy_calib(a, b, n_bytes / 16, 0)) rc = P2M_E_HIP;
+    if (!rc && hipEventRecord(e1, 0) != hipSuccess) rc = P2M_E_HIP;
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = P2M_E_HIP;
+    float t = 0.f;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = P2M_E_HIP;
+    if (!rc) *ms = (double)t / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(a); dfree(b);
+    (void)hipGetLastError();
+    return rc;
+}

@@ -124,6 +124,7 @@ void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t 
 void p2m_launch_f32_to_f64(const float *in, double *out, int64_t n, hipStream_t s);
 
 int p2m_launch_selftest_math(int op, const double *x, const double *y, double *out, int64_t n, hipStream_t s);
+int p2m_launch_copy_calib(const void *src, void *dst, int64_t n4, hipStream_t s);
 
 /* make_maptable kernel (p2m_maptable.hip) */
 int p2m_launch_make_maptable(int32_t model, double nu, double alpha_cmb, int32_t nchit, int32_t nmht, int32_t nrt,

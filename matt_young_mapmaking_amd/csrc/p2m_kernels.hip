@@ -1244,3 +1244,20 @@ int p2m_launch_selftest_math(int op, const double *x, const double *y, double *o
     hipLaunchKernelGGL(k_selftest_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, op, x, y, out, n);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+/* plain streaming copy, 16 B per lane per iteration: the known-traffic
+ * kernel that calibrates HBM byte counters and measures copy bandwidth */
+__global__ void __launch_bounds__(256) k_copy_calib(const float4 *__restrict__ src, float4 *__restrict__ dst, int64_t n4)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int p2m_launch_copy_calib(const void *src, void *dst, int64_t n4, hipStream_t s)
+{
+    if (n4 <= 0) return 0;
+    int64_t g = (n4 + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_copy_calib, dim3((unsigned)g), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n4);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

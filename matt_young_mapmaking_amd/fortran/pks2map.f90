@@ -165,6 +165,20 @@ program pks2map
      call check(p2m_set_run(ctx, run), 'p2m_set_run')
   end if
 
+  ! optional: the records as painted (after cut, scramble, centring), in .pksc layout
+  call get_environment_variable('P2M_DUMP_CATALOGUE', env)
+  if (len_trim(env) > 0) then
+     open(newunit=u, file=trim(env), access='stream', form='unformatted', status='replace', action='write', &
+          iostat=ios)
+     if (ios /= 0) call fail('cannot write '//trim(env), 3)
+     if (nh > 0) then
+        write(u) int(nh, 4), maxval(halos(7, 1:nh)), halos(:, 1:nh)
+     else
+        write(u) 0_4, 0.0
+     end if
+     close(u)
+  end if
+
   call get_environment_variable('P2M_TILE', env)
   if (len_trim(env) > 0) then
      block

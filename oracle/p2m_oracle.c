@@ -750,3 +750,66 @@ int p2mo_math(int op, const double *x, const double *y, double *out, int64_t n)
     }
     return 0;
 }
+
+
+/* pks2map.f90:144-262 on the host, default real (float).  Fortran's
+ * `2 * randnum - 1` and `2 * 3.14159 * randnum` mix real(4) and real(8):
+ * the float factor is formed first, the product is taken in double and
+ * rounded to float on assignment. */
+void p2mo_host_branches(float *h, int64_t *n, float cut_low, float cut_high, float chihview,
+                        int scramble, int centre)
+{
+    int64_t m = 0;
+    for (int64_t i = 0; i < *n; ++i) {                       /* :144-160 */
+        float *p = h + 7 * i;
+        float xh = p[2] - chihview, yh = p[1], zh = p[0];
+        float chih = sqrtf((xh * xh + yh * yh) + zh * zh);
+        if (chih > cut_high || chih < cut_low) continue;
+        if (m != i) memmove(h + 7 * m, p, 7 * sizeof(float));
+        ++m;
+    }
+    *n = m;
+    if (scramble == 1) {                                      /* :162-187 */
+        int32_t seed[4] = {13579, 13579, 13579, 13579};       /* seed = 13579*(myid+1), myid 0 */
+        const float twopi = 2.0f * 3.14159f;
+        for (int64_t i = 0; i < m; ++i) {
+            float *p = h + 7 * i;
+            float xh = p[0], yh = p[1], zh = p[2];
+            float chih = sqrtf((xh * xh + yh * yh) + zh * zh);
+            double r = p2mo_ranf(seed);
+            float muran = (float)(2.0 * r - 1.0);
+            r = p2mo_ranf(seed);
+            float phiran = (float)((double)twopi * r);
+            float rh = sqrtf(1.0f - muran * muran) * chih;
+            p[0] = rh * cosf(phiran);
+            p[1] = rh * sinf(phiran);
+            p[2] = muran * chih;
+        }
+    }
+    if (centre == 0 || m == 0) return;                        /* :189 */
+    float rthmax = h[6];
+    for (int64_t i = 1; i < m; ++i) if (h[7 * i + 6] > rthmax) rthmax = h[7 * i + 6];
+    float xm = -1e5f, ym = -1e5f, zm = -1e5f;
+    for (int64_t i = 0; i < m; ++i)                           /* :200-213: the last match wins */
+        if (h[7 * i + 6] == rthmax) { xm = h[7 * i + 2]; ym = h[7 * i + 1]; zm = h[7 * i + 0]; }
+    for (int64_t i = 0; i < m; ++i) {                         /* :220-222 */
+        if (xm < 0.0f) h[7 * i + 2] = -h[7 * i + 2];
+        if (ym < 0.0f) h[7 * i + 1] = -h[7 * i + 1];
+        if (zm < 0.0f) h[7 * i + 0] = -h[7 * i + 0];
+    }
+    xm = fabsf(xm); ym = fabsf(ym); zm = fabsf(zm);
+    const float rt = asinf(zm / sqrtf(xm * xm + zm * zm));   /* :227-229 */
+    const float xml = xm / fabsf(xm) * sqrtf(xm * xm + zm * zm);
+    const float rp = asinf(ym / sqrtf(ym * ym + xml * xml));
+    const float ct = cosf(rt), st = sinf(rt), cp = cosf(rp), sp = sinf(rp);
+    for (int64_t i = 0; i < m; ++i) {                         /* :232-244 */
+        float *p = h + 7 * i;
+        float xh = p[2] * ct + p[0] * st;
+        float zh = -p[2] * st + p[0] * ct;
+        float yh = -xh * sp + p[1] * cp;
+        xh = xh * cp + p[1] * sp;
+        p[2] = zh;
+        p[1] = yh;
+        p[0] = xh;
+    }
+}

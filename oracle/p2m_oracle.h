@@ -110,6 +110,14 @@ int p2mo_math(int op, const double *x, const double *y, double *out, int64_t n);
 /* 48-bit LCG of RandomField/random.f90:40-57 */
 double p2mo_ranf(int32_t seed[4]);
 
+/* Radial cut with compaction (pks2map.f90:144-160), scramble (:162-187) and
+ * centring on the largest halo (:189-262), in float arithmetic with the C
+ * library's float functions, on 7-float records in place.  *n is updated to
+ * the number kept.  Velocities are carried with their halo (the reference
+ * compacts only positions and Rth; quirk Q18). */
+void p2mo_host_branches(float *h, int64_t *n, float cut_low, float cut_high, float chihview,
+                        int scramble, int centre);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,9 @@ def lib(flavour="portable"):
     L.p2mo_math.argtypes = [C.c_int, C.POINTER(d), C.POINTER(d), C.POINTER(d), i64]
     L.p2mo_ranf.argtypes = [C.POINTER(i32)]
     L.p2mo_ranf.restype = d
+    L.p2mo_host_branches.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int64), C.c_float, C.c_float, C.c_float,
+                                     C.c_int, C.c_int]
+    L.p2mo_host_branches.restype = None
     _LIBS[name] = L
     return L
 
@@ -215,6 +218,16 @@ def make_maptable(model=1, nu=148.0, alpha=0.0, nchit=100, nmht=100, nrt=200, no
     L.p2mo_make_maptable(model, nu, alpha, nchit, nmht, nrt, lo, hi, _ptr(hdr_i, C.c_int32),
                          _ptr(hdr_f, C.c_float), _ptr(tab, C.c_float), nthreads)
     return hdr_i, hdr_f, tab
+
+
+def host_branches(halos, cut_low, cut_high, chihview=0.0, scramble=0, centre=0):
+    """Radial cut, scramble and centring of pks2map.f90:144-262 on a copy of
+    the (n, 7) float32 records; returns the kept records."""
+    h = np.ascontiguousarray(halos, np.float32).copy()
+    n = C.c_int64(h.shape[0])
+    lib().p2mo_host_branches(_ptr(h, C.c_float), C.byref(n), C.c_float(cut_low), C.c_float(cut_high),
+                             C.c_float(chihview), int(scramble), int(centre))
+    return h[:n.value].copy()
 
 
 def ranf(seed):

@@ -54,6 +54,13 @@ def test_no_device_is_an_error():
     assert e.value.code in (-8, -3)
     with pytest.raises(abi.P2MError):
         abi.make_maptable(nchit=3, nmht=3, nrt=3)
+    with pytest.raises(abi.P2MError) as e:
+        abi.selftest_copy(1 << 20)
+    assert e.value.code in (-8, -3)
+    for nb, reps in ((17, 1), (0, 1), (1 << 20, 0), ((1 << 34) + 16, 1)):
+        with pytest.raises(abi.P2MError) as e:
+            abi.selftest_copy(nb, reps)
+        assert e.value.code == -1
 
 
 def test_strerror_and_version():

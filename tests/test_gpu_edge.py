@@ -1,7 +1,8 @@
 """Edge cases on the GPU: empty and all-cut catalogues, discs centred on the
 poles (the a == 0 branch of discphirange_at_z), discs covering the pole or
 most of the sky, halos whose disc radius is invalid (r >= chi, NaN), the kSZ
-profile, and the drop-in Fortran executable end to end."""
+profile, and the drop-in Fortran executable end to end, with its scramble and
+centring branches."""
 import os
 import subprocess
 
@@ -116,3 +117,41 @@ def test_fortran_pks2map_end_to_end(table, tmp_path):
     nzf = ref["fs"] != 0
     assert np.max(np.abs(fs[nzf] / ref["fs"][nzf] - 1)) < 1e-6
     assert os.path.getsize(tmp_path / "out_dp.bin") == 8 + 40000
+
+
+@pytest.mark.parametrize("scramble,centre", [(1, 0), (0, 1), (1, 1)])
+def test_fortran_pks2map_scramble_and_centre(table, tmp_path, scramble, centre):
+    """The host branches of the drop-in (pks2map.f90:162-262): the records it
+    paints equal the oracle's restatement of the branches to float rounding,
+    and its map equals the oracle's map of those records."""
+    print(f"\n[gpu edge] pks2map scramble={scramble} centre={centre}", flush=True)
+    exe = os.path.join(ROOT, "matt_young_mapmaking_amd", "bin", "pks2map")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built")
+    h = catalog.config_catalog("C2", n=40000)
+    formats.write_pksc(tmp_path / "cat.pksc", h)
+    formats.write_table(tmp_path / "tab_148", *table)
+    env = dict(os.environ, P2M_DUMP_CATALOGUE=str(tmp_path / "painted.pksc"))
+    r = subprocess.run([exe, "cat.pksc", "out", "tab_148", "0.05", "512", str(scramble), str(centre), "128", "10",
+                        "1.2", "0", "1", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr
+    _, _, painted = formats.read_pksc(str(tmp_path / "painted.pksc"))
+    o = p2mo.Oracle(*table)
+    lo, hi = o.rofz(float(np.float32(0.05))), o.rofz(float(np.float32(1.2)))
+    expect = p2mo.host_branches(h, np.float32(lo), np.float32(hi), 0.0, scramble=scramble, centre=centre)
+    assert painted.shape == expect.shape
+    fin = np.all(np.isfinite(expect[:, :3]), axis=1)
+    assert np.array_equal(fin, np.all(np.isfinite(painted[:, :3]), axis=1))
+    assert (not scramble) or not fin[0]                  # Q10: the first scrambled halo is NaN
+    assert fin.sum() > 30000
+    chi = np.linalg.norm(expect[fin, :3].astype(np.float64), axis=1)
+    dev = np.max(np.abs(painted[fin, :3].astype(np.float64) - expect[fin, :3]), axis=1) / chi
+    assert np.max(dev) < 1e-5
+    assert np.array_equal(painted[:, 6], expect[:, 6])
+    # the map of the painted records (radial cut already applied: the band is widened so
+    # that float rounding of the rotated distances cannot drop a record)
+    ref = o.paint(painted[fin], nside=512, npix=128, zmin=0.04, zmax=1.3, fov_deg=10.0, nthreads=8)
+    ns, order, hp, _ = formats.read_healpix_fits(str(tmp_path / "out_hp.fits"))
+    nz = ref["hp"] != 0
+    assert np.count_nonzero((hp != 0) != nz) == 0
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) < 1e-6
