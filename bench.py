@@ -90,10 +90,11 @@ def main():
     ap.add_argument("--workload", default="C3")
     ap.add_argument("--n-halos", type=int, default=None)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--tile", default=None, help="W,WG tile shape (default: autotune)")
+    ap.add_argument("--tile", default="512,256", help="W,WG tile shape, or 'auto' for the library's autotuner "
+                    "(default: the shape the committed profiles cover)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sky-band "
@@ -114,7 +115,7 @@ def main():
     log("catalogue resident in HBM")
 
     # headline: FP32 + compensated tiled painter
-    tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+    tile = tuple(int(x) for x in args.tile.split(",")) if args.tile and args.tile != "auto" else None
     p32, sec32, st32, k32 = run_painter(dev, n, hi, hf, tab, abi.PREC_FP32_COMP, nside, npix, args.steps, args.warmup,
                                         tile=tile)
     hp32, _ = p32.maps(dtype=np.float32)
