@@ -42,8 +42,9 @@ namespace {
 
 const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
 
-struct Ctr {                 /* layout of the device counter block */
-    enum { N_OUT = 0, DEC = 1, N_FLAT = 5, N_CAND = 6, PAINT = 8, TOTAL = 16 };
+struct Ctr {                 /* layout of one shard row of the device counter block */
+    enum { DEC = 1, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, TOTAL = P2M_CTR_STRIDE,
+           ALL = P2M_CTR_STRIDE * P2M_SHARDS };
 };
 
 }  // namespace
@@ -102,6 +103,11 @@ struct p2m_ctx {
     int64_t stage_cap = 0;
     uint32_t *d_cand = nullptr;
     int64_t cand_cap = 0;
+    HaloRec *d_halo_s = nullptr;   /* painted halos in spatial order (tiled painters) */
+    TileHalo *d_hot = nullptr;     /* the same, packed for the FP32 painter */
+    uint32_t *d_keys = nullptr, *d_vals = nullptr;
+    void *d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     p2m_stats st;
 };
 
@@ -227,7 +233,7 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
             rc |= dmalloc((void **)&c->d_fs_hash, sizeof(unsigned long long) * c->fsnpix);
         }
     }
-    if (!rc) rc |= dmalloc((void **)&c->d_ctr, sizeof(unsigned long long) * Ctr::TOTAL);
+    if (!rc) rc |= dmalloc((void **)&c->d_ctr, sizeof(unsigned long long) * Ctr::ALL);
     if (rc) { p2m_destroy(c); return P2M_E_NOMEM; }
     *out = c;
     int e = p2m_clear(c);
@@ -248,6 +254,7 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
+    dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
     for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -439,6 +446,20 @@ static int scan_counts(p2m_ctx *c, const uint32_t *cnt, unsigned long long *off,
 
 /* The per-halo buffers exist after the first paint call even for an empty
  * catalogue: the binning writes the terminating ring count nr[n_halo]. */
+/* counter block summed over its shards (blocks until the stream is idle) */
+static int read_counters(p2m_ctx *c, unsigned long long out[Ctr::TOTAL], unsigned long long *raw = nullptr)
+{
+    unsigned long long hbuf[Ctr::ALL];
+    unsigned long long *h = raw ? raw : hbuf;
+    HIPCHK(hipMemcpyAsync(h, c->d_ctr, sizeof(hbuf), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < Ctr::TOTAL; ++k) {
+        out[k] = 0;
+        for (int sh = 0; sh < P2M_SHARDS; ++sh) out[k] += h[sh * Ctr::TOTAL + k];
+    }
+    return 0;
+}
+
 static int ensure_halo_cap(p2m_ctx *c, int64_t n)
 {
     if (n <= c->halo_cap && c->d_halo) return 0;
@@ -446,11 +467,21 @@ static int ensure_halo_cap(p2m_ctx *c, int64_t n)
     dfree(c->d_flat_idx); c->d_flat_idx = nullptr;
     dfree(c->d_nr); c->d_nr = nullptr;
     dfree(c->d_ring_off); c->d_ring_off = nullptr;
+    dfree(c->d_halo_s); c->d_halo_s = nullptr;
+    dfree(c->d_hot); c->d_hot = nullptr;
+    dfree(c->d_keys); c->d_keys = nullptr;
+    dfree(c->d_vals); c->d_vals = nullptr;
     int64_t cap = std::max<int64_t>(n, 1024);
     if (dmalloc((void **)&c->d_halo, sizeof(HaloRec) * cap)) return P2M_E_NOMEM;
     if (dmalloc((void **)&c->d_flat_idx, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
     if (dmalloc((void **)&c->d_nr, sizeof(uint32_t) * (cap + 1))) return P2M_E_NOMEM;
     if (dmalloc((void **)&c->d_ring_off, sizeof(unsigned long long) * (cap + 1))) return P2M_E_NOMEM;
+    if (c->cfg.precision != P2M_PREC_PLAIN_FP64 && c->cfg.nside > 0) {
+        if (dmalloc((void **)&c->d_halo_s, sizeof(HaloRec) * cap)) return P2M_E_NOMEM;
+        if (dmalloc((void **)&c->d_hot, sizeof(TileHalo) * cap)) return P2M_E_NOMEM;
+        if (dmalloc((void **)&c->d_keys, sizeof(uint32_t) * 2 * cap)) return P2M_E_NOMEM;
+        if (dmalloc((void **)&c->d_vals, sizeof(uint32_t) * 2 * cap)) return P2M_E_NOMEM;
+    }
     c->halo_cap = cap;
     return 0;
 }
@@ -467,10 +498,10 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
     a.fov = ((c->cfg.fov_deg / 360.0) * 2.0) * c->mypi;        /* pks2map.f90:92 */
     a.dpix = c->cfg.npix_fs > 0 ? a.fov / (double)c->cfg.npix_fs : 1.0;
     a.out = c->d_halo; a.cap = c->halo_cap;
-    a.n_out = c->d_ctr + Ctr::N_OUT;
     a.flat_idx = c->d_flat_idx;
     a.cand = c->d_cand;
     a.n_cand = c->d_ctr + Ctr::N_CAND;
+    a.n_cand0 = c->d_ctr + Ctr::N_CAND0;
     a.n_flat_out = c->d_ctr + Ctr::N_FLAT;
     a.counters = c->d_ctr + Ctr::DEC;
     p2m_launch_decode(a, c->stream);
@@ -512,9 +543,25 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
             if (allow_tune) { int rc = autotune(c, n_halo, n_flat); if (rc) return rc; }
             else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
         }
+        /* painted halos in spatial order: neighbouring halos share tiles */
+        {
+            const int32_t nphi = std::max(1, (4 * c->cfg.nside) >> c->log2w);
+            size_t need = 0;
+            if (p2m_sort_halos(c->d_halo, c->d_halo_s, c->d_hot, n_halo, c->cfg.nside, nphi, c->g.nrt, c->g.nmht,
+                               c->d_keys, c->d_vals, nullptr, &need, c->stream)) return P2M_E_HIP;
+            if (need > c->sort_tmp_bytes || !c->d_sort_tmp) {
+                dfree(c->d_sort_tmp); c->d_sort_tmp = nullptr; c->sort_tmp_bytes = 0;
+                if (dmalloc(&c->d_sort_tmp, need + 256)) return P2M_E_NOMEM;
+                c->sort_tmp_bytes = need + 256;
+            }
+            size_t bytes = c->sort_tmp_bytes;
+            if (p2m_sort_halos(c->d_halo, c->d_halo_s, c->d_hot, n_halo, c->cfg.nside, nphi, c->g.nrt, c->g.nmht,
+                               c->d_keys, c->d_vals, c->d_sort_tmp, &bytes, c->stream)) return P2M_E_HIP;
+        }
+        const HaloRec *hs = c->d_halo_s;
         BinArgs b;
         memset(&b, 0, sizeof(b));
-        b.halos = c->d_halo; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
+        b.halos = hs; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
         b.nr = c->d_nr; b.ring_off = c->d_ring_off;
         b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt; b.tile_fill = c->d_tile_fill;
         b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc;
@@ -537,8 +584,11 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
         unsigned long long nseg = 0, nlisted = 0;
         HIPCHK(hipMemcpyAsync(&nseg, c->d_tile_off + c->n_tiles, sizeof(nseg), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(&nlisted, pc + 0, sizeof(nlisted), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        {
+            unsigned long long hc[Ctr::TOTAL];
+            int rc = read_counters(c, hc); if (rc) return rc;
+            nlisted = hc[Ctr::PAINT + 0];
+        }
         if (mode == P2M_PREC_FP32_COMP) {
             /* deferred FP64 pixels: ~5e-4 of the listed pixels in practice */
             int64_t need = (int64_t)(nlisted / 16) + (1 << 20);
@@ -560,7 +610,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         HIPCHK(hipEventRecord(c->ev[3], c->stream));
         TileArgs t;
         memset(&t, 0, sizeof(t));
-        t.halos = c->d_halo; t.nside = c->cfg.nside; t.log2w = c->log2w; t.n_rings = c->n_rings;
+        t.halos = hs; t.hot = c->d_hot; t.nside = c->cfg.nside; t.log2w = c->log2w; t.n_rings = c->n_rings;
         t.n_maps = c->cfg.n_maps; t.profile = c->cfg.profile; t.accumulate = c->hp_zero ? 0 : 1;
         t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
         t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
@@ -580,7 +630,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         if (mode == P2M_PREC_FP32_COMP) {
             FallbackArgs f;
             memset(&f, 0, sizeof(f));
-            f.halos = c->d_halo; f.fb = c->d_fb; f.fb_count = pc + 6; f.fb_cap = c->fb_cap;
+            f.halos = hs; f.fb = c->d_fb; f.fb_count = pc + 6; f.fb_cap = c->fb_cap;
             f.nside = c->cfg.nside; f.n_maps = c->cfg.n_maps; f.profile = c->cfg.profile;
             f.g = c->g; f.n_flat = c->n_flat;
             for (int m = 0; m < c->cfg.n_maps; ++m) { f.tab_flat[m] = c->d_flat[m]; f.map32[m] = c->d_hp32[m]; }
@@ -600,24 +650,29 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
 
 static int finish_stats(p2m_ctx *c)
 {
-    unsigned long long h[Ctr::TOTAL];
-    HIPCHK(hipMemcpyAsync(h, c->d_ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long h[Ctr::TOTAL], raw[Ctr::ALL];
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = read_counters(c, h, raw);
+    if (rc) return rc;
     p2m_stats &s = c->st;
     s.n_cut = (int64_t)h[Ctr::DEC + 0];
-    s.n_painted = (int64_t)h[Ctr::DEC + 1];
     s.n_radius_err = (int64_t)h[Ctr::DEC + 2];
+    s.n_painted = (int64_t)h[Ctr::N_CAND] - s.n_radius_err;
     s.n_listed_hp = (int64_t)h[Ctr::PAINT + 0];
     s.n_upd_hp = (int64_t)h[Ctr::PAINT + 1];
     s.n_upd_fs = (int64_t)h[Ctr::PAINT + 2];
     s.n_ring_gaps = (int64_t)h[Ctr::PAINT + 3];
     s.n_segments = (int64_t)h[Ctr::PAINT + 4];
     s.n_fallback = (int64_t)h[Ctr::PAINT + 5];
-    if (c->cfg.precision == P2M_PREC_FP32_COMP && (int64_t)h[Ctr::PAINT + 6] > c->fb_cap) {
-        snprintf(g_hip_detail, sizeof(g_hip_detail), "deferred-pixel list overflow (%llu > %lld)",
-                 (unsigned long long)h[Ctr::PAINT + 6], (long long)c->fb_cap);
-        return P2M_E_NOMEM;
+    if (c->cfg.precision == P2M_PREC_FP32_COMP) {
+        for (int sh = 0; sh < P2M_SHARDS; ++sh) {
+            const unsigned long long v = raw[sh * Ctr::TOTAL + Ctr::PAINT + 6];
+            if ((int64_t)v > c->fb_cap / P2M_SHARDS) {
+                snprintf(g_hip_detail, sizeof(g_hip_detail), "deferred-pixel list overflow (%llu > %lld in sub-list %d)",
+                         v, (long long)(c->fb_cap / P2M_SHARDS), sh);
+                return P2M_E_NOMEM;
+            }
+        }
     }
     s.cut_low = c->cut_low; s.cut_high = c->cut_high;
     float ms = 0;
@@ -643,13 +698,13 @@ static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n,
     HIPCHK(hipSetDevice(c->cfg.device));
     int rc = ensure_halo_cap(c, n);
     if (rc) return rc;
-    p2m_launch_fill_u64(c->d_ctr, Ctr::TOTAL, c->stream);
+    p2m_launch_fill_u64(c->d_ctr, Ctr::ALL, c->stream);
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     const int64_t chunk = c->cfg.chunk_halos;
-    if (n > 0 && std::min(chunk, n) > c->cand_cap) {
+    if (n > 0 && n > c->cand_cap) {
         dfree(c->d_cand); c->d_cand = nullptr; c->cand_cap = 0;
-        if (dmalloc((void **)&c->d_cand, sizeof(uint32_t) * std::min(chunk, n))) return P2M_E_NOMEM;
-        c->cand_cap = std::min(chunk, n);
+        if (dmalloc((void **)&c->d_cand, sizeof(uint32_t) * n)) return P2M_E_NOMEM;
+        c->cand_cap = n;
     }
     if (!on_device && n > 0) {
         int64_t need = std::min(chunk, n) * 7;
@@ -670,10 +725,10 @@ static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n,
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    unsigned long long hc[Ctr::PAINT];
-    HIPCHK(hipMemcpyAsync(hc, c->d_ctr, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    int64_t n_halo = (int64_t)std::min<unsigned long long>(hc[Ctr::N_OUT], (unsigned long long)c->halo_cap);
+    unsigned long long hc[Ctr::TOTAL];
+    rc = read_counters(c, hc);
+    if (rc) return rc;
+    int64_t n_halo = (int64_t)std::min<unsigned long long>(hc[Ctr::N_CAND], (unsigned long long)c->halo_cap);
     int64_t n_flat = (int64_t)std::min<unsigned long long>(hc[Ctr::N_FLAT], (unsigned long long)c->halo_cap);
     rc = paint_halos(c, n_halo, n_flat, allow_tune);
     if (rc) return rc;
@@ -791,7 +846,7 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat)
         rc = p2m_clear(c); if (rc) return rc;
     }
     /* counters were polluted by the trials: reset the paint counters */
-    p2m_launch_fill_u64(c->d_ctr + Ctr::PAINT, Ctr::TOTAL - Ctr::PAINT, c->stream);
+    p2m_launch_clear_ctr(c->d_ctr, Ctr::PAINT, Ctr::TOTAL - Ctr::PAINT, c->stream);
     int rc = build_tiles(c, best_l); if (rc) return rc;
     c->wg = best_wg; c->tuned = true; c->hp_zero = true;
     return 0;

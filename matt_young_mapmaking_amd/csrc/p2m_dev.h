@@ -65,6 +65,18 @@ struct HaloRec {
     int32_t jmin, jmax, kmin, kmax;  /* flat-sky index box (haloproject.f90:124-128) */
 };
 
+/* The fields of a painted halo that the FP32 tile painter stages per
+ * segment, packed into one 64-B line (written in spatial order). */
+struct __attribute__((aligned(64))) TileHalo {
+    double zz0, s0, phi0, thr;   /* as in HaloRec */
+    float chi2;                  /* (float)(chi_r * chi_r) */
+    float vh;                    /* velocity factor; 0 when chi > chimaxt */
+    float fc, fm;                /* table fractions, rounded to float */
+    int32_t colbase;             /* nrt * ((imh - 1) + nmht * (ichi - 1)) */
+    uint32_t pad;
+    unsigned long long hash;     /* membership hash of the catalogue index */
+};
+
 /* ------------------------------------------------------------ cosmology */
 P2M_DEV double tab_lookup(const double *t, int n, double tmin, double dt, double x)
 {

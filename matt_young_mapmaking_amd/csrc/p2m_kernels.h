@@ -9,6 +9,14 @@
 #include <stdint.h>
 #include "p2m_dev.h"
 
+/* Statistics counters are sharded: shard k of counter c lives at
+ * c + k * P2M_CTR_STRIDE, and workgroups add into shard (block % P2M_SHARDS),
+ * so no address takes more than 1/64 of the atomics (same-address device
+ * atomics serialise beyond the L2).  The host sums the shards.  Append
+ * counters (whose value kernels read back) use shard 0 only. */
+#define P2M_SHARDS 64
+#define P2M_CTR_STRIDE 16
+
 struct DecodeArgs {
     const float *halos;          /* 7 float32 per halo */
     int64_t n, index_base;
@@ -18,14 +26,14 @@ struct DecodeArgs {
     int32_t precut;
     int32_t profile, nside, npix_fs;
     double fov, dpix;
-    uint32_t *cand;              /* chunk indices that passed both cuts */
-    unsigned long long *n_cand;
-    HaloRec *out;
+    uint32_t *cand;              /* chunk indices that passed both cuts, all chunks */
+    unsigned long long *n_cand;  /* append counter over all chunks of a call */
+    unsigned long long *n_cand0; /* its value before this chunk's cull */
+    HaloRec *out;                /* record k of candidate k (all chunks) */
     int64_t cap;
-    unsigned long long *n_out;   /* append counter */
     uint32_t *flat_idx;          /* slots of halos the flat-sky painter covers */
     unsigned long long *n_flat_out;
-    unsigned long long *counters;/* [0] passed radial cut, [1] painted, [2] radius errors */
+    unsigned long long *counters;/* sharded: [0] passed radial cut, [2] radius errors */
 };
 
 struct PaintArgs {
@@ -66,6 +74,7 @@ struct BinArgs {
 
 struct TileArgs {
     const HaloRec *halos;
+    const TileHalo *hot;          /* the same halos, packed for the FP32 painter */
     int32_t nside, log2w, n_rings, n_maps, profile, accumulate;
     int64_t n_tiles;
     const int32_t *tile_base;
@@ -83,9 +92,9 @@ struct TileArgs {
     float *map32[P2M_MAXMAPS];
     uint32_t *count;
     unsigned long long *hash;
-    uint4 *fb;                    /* deferred FP64 pixels: (ring, ip, halo, 0) */
-    unsigned long long *fb_count;
-    int64_t fb_cap;
+    uint4 *fb;                    /* deferred FP64 pixels: (ring, ip, halo, 0), P2M_SHARDS sub-lists */
+    unsigned long long *fb_count; /* sharded: sub-list k's length at fb_count[k * P2M_CTR_STRIDE] */
+    int64_t fb_cap;               /* total; each sub-list holds fb_cap / P2M_SHARDS */
     unsigned long long *counters; /* [1] hp updates, [5] FP64 fallbacks */
 };
 
@@ -108,12 +117,16 @@ void p2m_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void p2m_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
 void p2m_launch_fill_f64(double *p, int64_t n, hipStream_t s);
 void p2m_launch_fill_f32(float *p, int64_t n, hipStream_t s);
+void p2m_launch_clear_ctr(unsigned long long *ctr, int first, int count, hipStream_t s);
 void p2m_launch_decode(const DecodeArgs &a, hipStream_t s);
 void p2m_launch_paint_plain_hp(const PaintArgs &a, hipStream_t s);
 void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s);
 void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
+int p2m_sort_halos(const HaloRec *in, HaloRec *out, TileHalo *hot, int64_t n, int32_t nside, int32_t nphi,
+                   int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
+                   hipStream_t s);
 int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership);
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);

@@ -46,6 +46,17 @@ static inline int fill_grid(int64_t n) { int64_t g = (n + 255) / 256; return (in
 void p2m_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v); }
 void p2m_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(256), 0, s, p, n); }
 void p2m_launch_fill_f64(double *p, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(fill_grid(n)), dim3(256), 0, s, p, n); }
+__global__ void k_clear_ctr(unsigned long long *ctr, int first, int count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < P2M_SHARDS * count) ctr[(t / count) * P2M_CTR_STRIDE + first + t % count] = 0ULL;
+}
+
+void p2m_launch_clear_ctr(unsigned long long *ctr, int first, int count, hipStream_t s)
+{
+    if (count > 0) hipLaunchKernelGGL(k_clear_ctr, dim3((P2M_SHARDS * count + 255) / 256), dim3(256), 0, s, ctr, first, count);
+}
+
 void p2m_launch_fill_f32(float *p, int64_t n, hipStream_t s) { if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3(fill_grid(n)), dim3(256), 0, s, p, n); }
 
 /* wave-aggregated append: returns the slot of this lane (or garbage if !take) */
@@ -64,12 +75,14 @@ __device__ static inline unsigned long long wave_append(unsigned long long *coun
 
 __device__ static inline void wave_count(unsigned long long *counter, unsigned long long v)
 {
-    /* sum v over the wave, one atomic */
+    /* sum v over the wave, one atomic into this wave's shard of a statistics counter */
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (__lane_id() == 0 && v) atomicAdd(counter, v);
+    const unsigned shard = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % P2M_SHARDS;
+    if (__lane_id() == 0 && v) atomicAdd(counter + shard * P2M_CTR_STRIDE, v);
 }
 
-/* sum v over the workgroup, one atomic per workgroup; every thread calls */
+/* sum v over the workgroup, one atomic per workgroup into its shard of a
+ * statistics counter; every thread calls */
 __device__ static inline void block_count(unsigned long long *counter, unsigned long long v, unsigned long long *lds16)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -80,7 +93,7 @@ __device__ static inline void block_count(unsigned long long *counter, unsigned 
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += lds16[k];
-        if (t) atomicAdd(counter, t);
+        if (t) atomicAdd(counter + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE, t);
     }
 }
 
@@ -88,55 +101,58 @@ __device__ static inline void block_count(unsigned long long *counter, unsigned 
 /* decode                                                                 */
 /* ===================================================================== */
 
-/* workgroup-level append: returns this thread's slot (valid if take);
- * one atomic per workgroup; every thread calls */
-__device__ static inline unsigned long long block_append(unsigned long long *counter, bool take,
-                                                         unsigned int *wcnt, unsigned long long *base_s)
-{
-    const unsigned long long mask = __ballot(take);
-    const int lane = __lane_id(), wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    if (lane == 0) wcnt[wv] = (unsigned int)__popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int t = 0;
-        for (int k = 0; k < nwv; ++k) t += wcnt[k];
-        *base_s = t ? atomicAdd(counter, (unsigned long long)t) : 0ULL;
-    }
-    __syncthreads();
-    unsigned long long off = *base_s;
-    for (int k = 0; k < wv; ++k) off += wcnt[k];
-    const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-    off += (unsigned long long)__popcll(below);
-    __syncthreads();
-    return off;
-}
+#define CULL_SPAN 8192
 
 /* Cull pass over every record: the radial cut (pks2map.f90:144-160) and the
- * mass cut on the unconverted mass (:264, :296).  Survivors' chunk indices
- * are appended to `cand`; the FP64 setup then runs on full waves. */
+ * mass cut on the unconverted mass (:264, :296).  A workgroup takes spans of
+ * CULL_SPAN consecutive records, collects the survivors' chunk indices in
+ * LDS (wave-aggregated LDS atomics), then reserves their range with one
+ * global atomic per span and copies them out coalesced.  The FP64 setup
+ * then runs on full waves of survivors. */
 __global__ void __launch_bounds__(256) k_cull(DecodeArgs A)
 {
     const CosmoDev &c = A.c;
-    __shared__ unsigned long long lds16[16];
-    __shared__ unsigned int wcnt[16];
+    __shared__ uint32_t buf[CULL_SPAN];
+    __shared__ unsigned int nbuf;
     __shared__ unsigned long long base_s;
+    __shared__ unsigned long long lds16[16];
+    const double mconst = ((4.0 / 3.0) * c.mypi) * c.rho_0;
     unsigned long long tot_cut = 0;
-    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < A.n; i0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = i0 + threadIdx.x;
-        bool take = false;
-        if (i < A.n) {
-            const float *p = A.halos + 7 * i;
-            const double xh = (double)p[2], yh = (double)p[1], zh = (double)p[0];
-            const double xs = xh - A.chihview;
-            const double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
-            if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
-                tot_cut++;
-                const double mass = (((4.0 / 3.0) * c.mypi) * c.rho_0) * cube((double)p[6]);
-                take = !(mass < A.mmin);
+    const int64_t nspan = (A.n + CULL_SPAN - 1) / CULL_SPAN;
+    const int lane = __lane_id();
+    const unsigned long long below = lane == 0 ? 0ULL : ((~0ULL) >> (64 - lane));
+    for (int64_t sp = blockIdx.x; sp < nspan; sp += gridDim.x) {
+        if (threadIdx.x == 0) nbuf = 0;
+        __syncthreads();
+        const int64_t i0 = sp * CULL_SPAN;
+        for (int k = threadIdx.x; k < CULL_SPAN; k += blockDim.x) {
+            const int64_t i = i0 + k;
+            bool take = false;
+            if (i < A.n) {
+                const float *p = A.halos + 7 * i;
+                const double xh = (double)p[2], yh = (double)p[1], zh = (double)p[0];
+                const double xs = xh - A.chihview;
+                const double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
+                if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
+                    tot_cut++;
+                    const double mass = mconst * cube((double)p[6]);
+                    take = !(mass < A.mmin);
+                }
+            }
+            const unsigned long long mask = __ballot(take);
+            if (mask) {
+                unsigned int off = 0;
+                if (lane == 0) off = atomicAdd(&nbuf, (unsigned int)__popcll(mask));
+                off = __shfl(off, 0);
+                if (take) buf[off + (unsigned int)__popcll(mask & below)] = (uint32_t)i;
             }
         }
-        const unsigned long long slot = block_append(A.n_cand, take, wcnt, &base_s);
-        if (take) A.cand[slot] = (uint32_t)i;
+        __syncthreads();
+        const unsigned int cnt = nbuf;
+        if (threadIdx.x == 0 && cnt) base_s = atomicAdd(A.n_cand, (unsigned long long)cnt);
+        __syncthreads();
+        for (unsigned int k = threadIdx.x; k < cnt; k += blockDim.x) A.cand[base_s + k] = buf[k];
+        __syncthreads();
     }
     block_count(&A.counters[0], tot_cut, lds16);
 }
@@ -151,16 +167,14 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
     const CosmoDev &c = A.c;
     const TabGeom &g = A.g;
     __shared__ unsigned long long lds16[16];
-    __shared__ unsigned int wcnt[16];
-    __shared__ unsigned long long base_s;
-    unsigned long long tot_rad = 0, tot_painted = 0;
-    const unsigned long long ncand = *A.n_cand;
-    for (unsigned long long k0 = blockIdx.x * (unsigned long long)blockDim.x; k0 < ncand;
+    unsigned long long tot_rad = 0;
+    const unsigned long long kbeg = *A.n_cand0, kend = *A.n_cand;
+    for (unsigned long long k0 = kbeg + blockIdx.x * (unsigned long long)blockDim.x; k0 < kend;
          k0 += (unsigned long long)gridDim.x * blockDim.x) {
         const unsigned long long k = k0 + threadIdx.x;
         bool painted = false;
         HaloRec h;
-        if (k < ncand) {
+        if (k < kend) {
             const int64_t i = A.cand[k];
             const float *p = A.halos + 7 * i;
             const float p0 = p[0], p1 = p[1], p2 = p[2], p6 = p[6];
@@ -182,6 +196,10 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
             const double dtheta = 4.0 * p2m_asin(rh / chih);                      /* haloproject.f90:47 */
             if (!(dtheta >= 0.0 && dtheta <= HPX_PI)) {
                 tot_rad++;   /* pixel_routines.F90:1962 would abort */
+                /* an inert record: no rings, no flat-sky box */
+                h = HaloRec();
+                h.dtheta = -1.0; h.irmin = 0; h.irmax = -1; h.flat = 0;
+                h.cat = A.index_base + i;
             } else {
                 painted = true;
                 const double ux = xs / chih, uy = yh / chih, uz = zh / chih;
@@ -247,27 +265,29 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
                 }
             }
         }
-        const unsigned long long off = block_append(A.n_out, painted, wcnt, &base_s);
-        if (painted && off < (unsigned long long)A.cap) {
-            A.out[off] = h;
-            if (h.flat) {
-                const unsigned long long fs = atomicAdd(A.n_flat_out, 1ULL);
-                A.flat_idx[fs] = (uint32_t)off;
-            }
-        }
-        tot_painted += painted ? 1ULL : 0ULL;
+        const bool flat = k < kend && painted && h.flat;
+        if (k < kend && k < (unsigned long long)A.cap) A.out[k] = h;
+        const unsigned long long fs = wave_append(A.n_flat_out, flat);
+        if (flat) A.flat_idx[fs] = (uint32_t)k;
     }
     block_count(&A.counters[2], tot_rad, lds16);
-    block_count(&A.counters[1], tot_painted, lds16);
+}
+
+/* n_cand0 <- n_cand: the first candidate slot of the next chunk */
+__global__ void k_mark_cand(DecodeArgs A)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *A.n_cand0 = *A.n_cand;
 }
 
 void p2m_launch_decode(const DecodeArgs &a, hipStream_t s)
 {
     if (a.n <= 0) return;
+    int64_t gc = (a.n + CULL_SPAN - 1) / CULL_SPAN;
+    if (gc > 2048) gc = 2048;
     int64_t g = (a.n + 255) / 256;
-    if (g > 8192) g = 8192;          /* grid-stride: bounded number of append atomics */
-    p2m_launch_fill_u64(a.n_cand, 1, s);
-    hipLaunchKernelGGL(k_cull, dim3((unsigned)g), dim3(256), 0, s, a);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_mark_cand, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_cull, dim3((unsigned)gc), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_setup, dim3((unsigned)g), dim3(256), 0, s, a);
 }
 
@@ -470,29 +490,118 @@ __global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
     }
 }
 
-/* per (halo, ring): the query_disc pixel run (stored for emit) and its tile counts */
+/* Tile counts and segment slots are aggregated per workgroup in an LDS
+ * hash table keyed by tile: a chunk of BIN_CH consecutive (halo, ring)
+ * items touches few distinct tiles once the halos are in spatial order, so
+ * one global atomic per (chunk, tile) replaces one per segment.  A piece
+ * whose tile finds no hash slot within BIN_PROBE probes, or that finds the
+ * piece buffer full, takes the direct path (one global atomic). */
+#define BIN_CH 1024
+#define BIN_HASH 2048
+#define BIN_PROBE 32
+#define BIN_PIECES 2048
+#define BIN_EMPTY 0xffffffffu
+
+struct BinHash {
+    uint32_t key[BIN_HASH];
+    uint32_t cnt[BIN_HASH];
+    unsigned int npieces;
+};
+
+struct BinLDS : BinHash {
+    uint2 val[BIN_PIECES];       /* segment (halo, lo<<16|len) */
+    uint32_t where[BIN_PIECES];  /* hash slot << 16 | rank within (chunk, tile); BIN_EMPTY: none */
+};
+
+__device__ static inline int bin_slot(BinHash *L, uint32_t tile)
+{
+    uint32_t h = (tile * 2654435761u) >> 21;            /* 11 bits: BIN_HASH = 2048 */
+    for (int k = 0; k < BIN_PROBE; ++k) {
+        const uint32_t prev = atomicCAS(&L->key[h], BIN_EMPTY, tile);
+        if (prev == BIN_EMPTY || prev == tile) return (int)h;
+        h = (h + 1) & (BIN_HASH - 1);
+    }
+    return -1;
+}
+
+__device__ static inline void bin_reset(BinHash *L)
+{
+    for (int k = threadIdx.x; k < BIN_HASH; k += blockDim.x) { L->key[k] = BIN_EMPTY; L->cnt[k] = 0; }
+    if (threadIdx.x == 0) L->npieces = 0;
+}
+
+/* the items [start, end) of one wave, with their halo and ring offset:
+ * f(item, halo, k) for each item of this lane (items of a 64-group share
+ * at most 64 halos: every halo has at least one ring) */
+template <typename F>
+__device__ static inline void wave_range(const unsigned long long *off, int64_t n_halo, unsigned long long start,
+                                         unsigned long long end, F f)
+{
+    if (start >= end) return;
+    const int lane = __lane_id();
+    int64_t lo = 0, hi = n_halo - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= start) lo = mid; else hi = mid - 1;
+    }
+    int64_t h = lo;
+    for (unsigned long long c = start; c < end; c += 64) {
+        while (h + 1 < n_halo && off[h + 1] <= c) ++h;
+        const unsigned long long item = c + lane;
+        const int64_t hl = h + lane;
+        const unsigned long long my_off = hl <= n_halo ? off[hl] : ~0ULL;
+        int j = 0;
+        for (int step = 32; step > 0; step >>= 1) {
+            unsigned long long o = __shfl(my_off, j + step);
+            if (o <= item) j += step;
+        }
+        const unsigned long long hoff = __shfl(my_off, j);
+        if (item < end) f(item, h + j, item - hoff);
+        h += __shfl(j, 63);
+    }
+}
+
+/* per (halo, ring) item: the query_disc pixel run (stored for emit) and its tile counts */
 __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
 {
+    __shared__ BinHash L;
     __shared__ unsigned long long lds16[16];
     unsigned long long n_list = 0, n_seg = 0;
-    wave_items(B.ring_off, B.n_halo, B.n_items, [&](unsigned long long item, int64_t hi, unsigned long long k) {
-        const HaloRec &h = B.halos[hi];
-        const int32_t ring = h.irmin + (int32_t)k;
-        int32_t lo = 0, np = 0;
-        int32_t npr, kshift; int64_t first;
-        if (ring <= h.irmax) {
-            ring_info(B.nside, ring, npr, kshift, first);
-            np = disc_ring_run(h, B.nside, ring, npr, kshift, lo);
-        }
-        B.runs[item] = make_uint2((uint32_t)lo, (uint32_t)np);
-        if (np > 0) {
-            n_list += (unsigned long long)np;
-            run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t, int32_t) {
-                atomicAdd(&B.tile_cnt[tile], 1u);
-                n_seg++;
-            });
-        }
-    });
+    const unsigned long long nchunk = (B.n_items + BIN_CH - 1) / BIN_CH;
+    bin_reset(&L);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const unsigned long long per = BIN_CH / nwv;
+    for (unsigned long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        unsigned long long a = ch * BIN_CH + wv * per, e = a + per;
+        if (e > B.n_items) e = B.n_items;
+        wave_range(B.ring_off, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+            const HaloRec &h = B.halos[hi];
+            const int32_t ring = h.irmin + (int32_t)k;
+            int32_t lo = 0, np = 0;
+            int32_t npr, kshift; int64_t first;
+            if (ring <= h.irmax) {
+                ring_info(B.nside, ring, npr, kshift, first);
+                np = disc_ring_run(h, B.nside, ring, npr, kshift, lo);
+            }
+            B.runs[item] = make_uint2((uint32_t)lo, (uint32_t)np);
+            if (np > 0) {
+                n_list += (unsigned long long)np;
+                run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t, int32_t) {
+                    const int sl = bin_slot(&L, (uint32_t)tile);
+                    if (sl >= 0) atomicAdd(&L.cnt[sl], 1u);
+                    else atomicAdd(&B.tile_cnt[tile], 1u);
+                    n_seg++;
+                });
+            }
+        });
+        __syncthreads();
+        for (int k = threadIdx.x; k < BIN_HASH; k += blockDim.x)
+            if (L.key[k] != BIN_EMPTY) atomicAdd(&B.tile_cnt[L.key[k]], L.cnt[k]);
+        __syncthreads();
+        bin_reset(&L);
+        __syncthreads();
+    }
     block_count(&B.counters[0], n_list, lds16);
     block_count(&B.counters[4], n_seg, lds16);
 }
@@ -514,19 +623,128 @@ __global__ void __launch_bounds__(256) k_ring_gaps(BinArgs B)
 
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
-    wave_items(B.ring_off, B.n_halo, B.n_items, [&](unsigned long long item, int64_t hi, unsigned long long k) {
-        const uint2 run = B.runs[item];
-        if (run.y == 0) return;
-        const HaloRec &h = B.halos[hi];
-        const int32_t ring = h.irmin + (int32_t)k;
-        int32_t npr, kshift; int64_t first;
-        ring_info(B.nside, ring, npr, kshift, first);
-        run_pieces((int32_t)run.x, (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
-                   [&](int32_t tile, int32_t lo, int32_t len) {
-            const uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
-            B.segs[B.tile_off[tile] + slot] = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
+    __shared__ BinLDS L;
+    const unsigned long long nchunk = (B.n_items + BIN_CH - 1) / BIN_CH;
+    bin_reset(&L);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const unsigned long long per = BIN_CH / nwv;
+    for (unsigned long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        unsigned long long a = ch * BIN_CH + wv * per, e = a + per;
+        if (e > B.n_items) e = B.n_items;
+        wave_range(B.ring_off, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+            const uint2 run = B.runs[item];
+            if (run.y == 0) return;
+            const HaloRec &h = B.halos[hi];
+            const int32_t ring = h.irmin + (int32_t)k;
+            int32_t npr, kshift; int64_t first;
+            ring_info(B.nside, ring, npr, kshift, first);
+            run_pieces((int32_t)run.x, (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
+                       [&](int32_t tile, int32_t lo, int32_t len) {
+                const uint2 v = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
+                const unsigned int pi = atomicAdd(&L.npieces, 1u);
+                int sl = -1;
+                if (pi < BIN_PIECES) {
+                    sl = bin_slot(&L, (uint32_t)tile);
+                    if (sl >= 0) {
+                        const uint32_t rank = atomicAdd(&L.cnt[sl], 1u);
+                        L.val[pi] = v;
+                        L.where[pi] = ((uint32_t)sl << 16) | rank;
+                    } else {
+                        L.where[pi] = BIN_EMPTY;
+                    }
+                }
+                if (sl < 0) {       /* direct path */
+                    const uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
+                    B.segs[B.tile_off[tile] + slot] = v;
+                }
+            });
         });
-    });
+        __syncthreads();
+        /* reserve each (chunk, tile) range; cnt[] then holds its first slot */
+        for (int k = threadIdx.x; k < BIN_HASH; k += blockDim.x)
+            if (L.key[k] != BIN_EMPTY) L.cnt[k] = atomicAdd(&B.tile_fill[L.key[k]], L.cnt[k]);
+        __syncthreads();
+        const unsigned int np = L.npieces < BIN_PIECES ? L.npieces : BIN_PIECES;
+        for (unsigned int p = threadIdx.x; p < np; p += blockDim.x) {
+            const uint32_t w = L.where[p];
+            if (w == BIN_EMPTY) continue;
+            const uint32_t sl = w >> 16, rank = w & 0xffffu;
+            const uint32_t tile = L.key[sl];
+            B.segs[B.tile_off[tile] + L.cnt[sl] + rank] = L.val[p];
+        }
+        __syncthreads();
+        bin_reset(&L);
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* spatial order of the painted halos                                  */
+/* ------------------------------------------------------------------ */
+
+/* key = (band of 8 rings of the disc centre) * nphi + (azimuth bin of the
+ * centre): halos with equal or adjacent keys share ring tiles.  Inert
+ * records (no rings) sort last. */
+__global__ void __launch_bounds__(256) k_halo_keys(const HaloRec *h, int64_t n, int32_t nside, int32_t nphi,
+                                                   uint32_t key_max, uint32_t *keys, uint32_t *vals)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const HaloRec &r = h[i];
+        uint32_t key = key_max;
+        if (r.irmax >= r.irmin) {
+            const int32_t rc = ring_num(nside, r.zz0);
+            double ph = r.phi0;
+            if (ph < 0.0) ph += HPX_TWOPI;
+            int32_t pb = (int32_t)(ph * (double)nphi / HPX_TWOPI);
+            pb = pb < 0 ? 0 : (pb >= nphi ? nphi - 1 : pb);
+            key = (uint32_t)((rc - 1) >> 3) * (uint32_t)nphi + (uint32_t)pb;
+        }
+        keys[i] = key;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const uint32_t *order, int64_t n, HaloRec *out,
+                                                      TileHalo *hot, int32_t nrt, int32_t nmht)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const HaloRec h = in[order[i]];
+        out[i] = h;
+        TileHalo t;
+        t.zz0 = h.zz0; t.s0 = h.s0; t.phi0 = h.phi0; t.thr = h.thr;
+        t.chi2 = (float)(h.chi_r * h.chi_r);
+        t.vh = h.chi_out ? 0.0f : (float)h.vh;
+        t.fc = (float)h.fc; t.fm = (float)h.fm;
+        t.colbase = nrt * ((h.imh - 1) + nmht * (h.ichi - 1));
+        t.pad = 0;
+        t.hash = halo_hash(h.cat);
+        hot[i] = t;
+    }
+}
+
+int p2m_sort_halos(const HaloRec *in, HaloRec *out, TileHalo *hot, int64_t n, int32_t nside, int32_t nphi,
+                   int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
+                   hipStream_t s)
+{
+    /* keys and vals hold 2n entries each: input then output halves */
+    const uint32_t nband = (uint32_t)((4 * nside - 2) >> 3) + 1;
+    const uint32_t key_max = nband * (uint32_t)nphi;
+    int bits = 1;
+    while ((1u << bits) <= key_max && bits < 32) ++bits;
+    if (!tmp) {
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, *tmp_bytes, keys, keys + n, vals, vals + n,
+                                                          (int)n, 0, bits, s);
+        return e == hipSuccess ? 0 : -1;
+    }
+    if (n <= 0) return 0;
+    unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_halo_keys, dim3(g), dim3(256), 0, s, in, n, nside, nphi, key_max, keys, vals);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, keys, keys + n, vals, vals + n,
+                                                      (int)n, 0, bits, s);
+    if (e != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_gather_halos, dim3(g), dim3(256), 0, s, in, vals + n, n, out, hot, nrt, nmht);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 static unsigned bin_grid(int64_t n) { int64_t g = (n + 255) / 256; return (unsigned)(g > 8192 ? 8192 : (g < 1 ? 1 : g)); }
@@ -534,16 +752,21 @@ void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ring_counts, dim3(bin_grid(b.n_halo + 1)), dim3(256), 0, s, b);
 }
+static unsigned chunk_grid(unsigned long long n_items)
+{
+    unsigned long long g = (n_items + BIN_CH - 1) / BIN_CH;
+    return (unsigned)(g > 2048 ? 2048 : (g < 1 ? 1 : g));
+}
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s)
 {
     if (b.n_items == 0) return;
-    hipLaunchKernelGGL(k_seg_count, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_seg_count, dim3(chunk_grid(b.n_items)), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_ring_gaps, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
 }
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s)
 {
     if (b.n_items == 0) return;
-    hipLaunchKernelGGL(k_seg_emit, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_seg_emit, dim3(chunk_grid(b.n_items)), dim3(256), 0, s, b);
 }
 
 /* exclusive scan of uint32 counts into uint64 offsets with a 64-bit accumulator */
@@ -606,7 +829,14 @@ __global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
     SegLDS *sl = (SegLDS *)(smem_raw + acc_bytes + mem_bytes);
 
     unsigned long long n_upd = 0, n_fb = 0;
-    for (int64_t t = blockIdx.x; t < A.n_tiles; t += gridDim.x) {
+    /* XCD-aware order: workgroup b runs on XCD b % 8; the workgroups of one
+     * XCD sweep one contiguous eighth of the tiles (ring-major), so tiles
+     * in flight on an XCD are neighbours that share halos in its L2 */
+    const bool xcd = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
+    const int64_t tstep = xcd ? gridDim.x / 8 : gridDim.x;
+    const int64_t tbeg = xcd ? (int64_t)(blockIdx.x % 8) * A.n_tiles / 8 + blockIdx.x / 8 : blockIdx.x;
+    const int64_t tend = xcd ? (int64_t)(blockIdx.x % 8 + 1) * A.n_tiles / 8 : A.n_tiles;
+    for (int64_t t = tbeg; t < tend; t += tstep) {
         /* ring of this tile: largest r with tile_base[r] <= t */
         int lo_r = 0, hi_r = A.n_rings - 1;
         while (lo_r < hi_r) {
@@ -890,10 +1120,21 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     const int nrt = A.g.nrt;
     const int64_t dmh = nrt, dch = (int64_t)nrt * A.g.nmht;
     unsigned long long n_upd = 0, n_fb = 0;
+    /* deferred pixels go to this workgroup's shard of the list */
+    const int64_t fbs = A.fb_cap / P2M_SHARDS;
+    unsigned long long *const fbc = A.fb_count + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE;
+    uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
     if (threadIdx.x == 0) sl->nfb = 0;
     __syncthreads();
 
-    for (int64_t t = blockIdx.x; t < A.n_tiles; t += gridDim.x) {
+    /* XCD-aware order: workgroup b runs on XCD b % 8; the workgroups of one
+     * XCD sweep one contiguous eighth of the tiles (ring-major), so tiles
+     * in flight on an XCD are neighbours that share halos in its L2 */
+    const bool xcd = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
+    const int64_t tstep = xcd ? gridDim.x / 8 : gridDim.x;
+    const int64_t tbeg = xcd ? (int64_t)(blockIdx.x % 8) * A.n_tiles / 8 + blockIdx.x / 8 : blockIdx.x;
+    const int64_t tend = xcd ? (int64_t)(blockIdx.x % 8 + 1) * A.n_tiles / 8 : A.n_tiles;
+    for (int64_t t = tbeg; t < tend; t += tstep) {
         int lo_r = 0, hi_r = A.n_rings - 1;
         while (lo_r < hi_r) {
             int mid = (lo_r + hi_r + 1) >> 1;
@@ -925,7 +1166,7 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
             const int tid = threadIdx.x;
             if (tid < nb) {
                 uint2 sg = A.segs[b0 + tid];
-                const HaloRec &h = A.halos[sg.x];
+                const TileHalo &h = A.hot[sg.x];
                 sl->halo[tid] = sg.x;
                 sl->lo[tid] = sg.y >> 16;
                 len = sg.y & 0xffffu;
@@ -934,16 +1175,16 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 sl->B[tid] = (float)((4.0 * sth) * h.s0);
                 sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
                 sl->thr_lo[tid] = (float)(h.thr * (1.0 - (double)A.band_c));
-                sl->chi2[tid] = (float)(h.chi_r * h.chi_r);
-                sl->vh[tid] = h.chi_out ? 0.0f : (float)h.vh;
-                float fc = (float)h.fc, fm = (float)h.fm;
+                sl->chi2[tid] = h.chi2;
+                sl->vh[tid] = h.vh;
+                const float fc = h.fc, fm = h.fm;
                 sl->w00[tid] = (1.0f - fc) * (1.0f - fm);
                 sl->w10[tid] = fc * (1.0f - fm);
                 sl->w01[tid] = (1.0f - fc) * fm;
                 sl->w11[tid] = fc * fm;
-                sl->colbase[tid] = nrt * ((h.imh - 1) + A.g.nmht * (h.ichi - 1));
+                sl->colbase[tid] = h.colbase;
                 sl->dphi0[tid] = off * kphi - h.phi0;
-                sl->hh[tid] = halo_hash(h.cat);
+                sl->hh[tid] = h.hash;
             }
             uint32_t v = len;
             const int lane = __lane_id();
@@ -1025,8 +1266,8 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                     uint4 e = make_uint4((uint32_t)ring, (uint32_t)ip, sl->halo[s], 0u);
                     if (slot < FB_LDS) sl->fb[slot] = e;
                     else {
-                        unsigned long long gs = atomicAdd(A.fb_count, 1ULL);
-                        if (gs < (unsigned long long)A.fb_cap) A.fb[gs] = e;
+                        unsigned long long gs = atomicAdd(fbc, 1ULL);
+                        if (gs < (unsigned long long)fbs) fbr[gs] = e;
                     }
                     n_fb++;
                     continue;
@@ -1064,10 +1305,10 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
         /* flush the deferred-pixel buffer when half full */
         if (sl->nfb >= FB_LDS / 2) {
             uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
-            if (threadIdx.x == 0) fb_base = atomicAdd(A.fb_count, (unsigned long long)nf);
+            if (threadIdx.x == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x)
-                if (fb_base + k < (unsigned long long)A.fb_cap) A.fb[fb_base + k] = sl->fb[k];
+                if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
             __syncthreads();
             if (threadIdx.x == 0) sl->nfb = 0;
         }
@@ -1076,10 +1317,10 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     {
         uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
         if (nf) {
-            if (threadIdx.x == 0) fb_base = atomicAdd(A.fb_count, (unsigned long long)nf);
+            if (threadIdx.x == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x)
-                if (fb_base + k < (unsigned long long)A.fb_cap) A.fb[fb_base + k] = sl->fb[k];
+                if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
         }
     }
     block_count(&A.counters[1], n_upd, lds16);
@@ -1092,12 +1333,15 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
 __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
 {
     __shared__ unsigned long long lds16[16];
-    unsigned long long n = *A.fb_count;
-    if (n > (unsigned long long)A.fb_cap) n = (unsigned long long)A.fb_cap;
+    /* blockIdx.y: shard of the deferred list */
+    const unsigned long long fbs = (unsigned long long)(A.fb_cap / P2M_SHARDS);
+    unsigned long long n = A.fb_count[blockIdx.y * P2M_CTR_STRIDE];
+    if (n > fbs) n = fbs;
+    const uint4 *fbr = A.fb + blockIdx.y * fbs;
     unsigned long long n_upd = 0;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
          i += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint4 e = A.fb[i];
+        const uint4 e = fbr[i];
         const int32_t ring = (int32_t)e.x, ip = (int32_t)e.y;
         const HaloRec &h = A.halos[e.z];
         int32_t npr, kshift; int64_t first;
@@ -1126,10 +1370,10 @@ __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
 
 int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)
 {
-    int64_t g = (n_max + 255) / 256;
+    int64_t g = (n_max / P2M_SHARDS + 255) / 256;
     if (g < 1) g = 1;
-    if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_fallback, dim3((unsigned)g), dim3(256), 0, s, a);
+    if (g > 64) g = 64;
+    hipLaunchKernelGGL(k_fallback, dim3((unsigned)g, P2M_SHARDS), dim3(256), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
