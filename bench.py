@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--tile", default="512,256", help="W,WG tile shape, or 'auto' for the library's autotuner "
+    ap.add_argument("--tile", default="1024,256", help="W,WG tile shape, or 'auto' for the library's autotuner "
                     "(default: the shape the committed profiles cover)")
     args = ap.parse_args()
     if args.gpus != 1:

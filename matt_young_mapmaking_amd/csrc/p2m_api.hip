@@ -91,6 +91,12 @@ struct p2m_ctx {
     bool tuned = false;
     std::vector<int32_t> tile_base;
     int32_t *d_tile_base = nullptr;
+    int32_t *d_tile_ring = nullptr;
+    uint32_t *d_group = nullptr;    /* halo of every 64th (halo, ring) item */
+    int64_t group_cap = 0;
+    uint32_t *d_fs_cnt = nullptr;   /* flat-sky box pixels per listed halo, and their scan */
+    unsigned long long *d_fs_off = nullptr;
+    int64_t fs_cap = 0;
     int64_t n_tiles = 0;
     int n_rings = 0;
     uint32_t *d_tile_cnt = nullptr, *d_tile_fill = nullptr;
@@ -103,8 +109,10 @@ struct p2m_ctx {
     int64_t stage_cap = 0;
     uint32_t *d_cand = nullptr;
     int64_t cand_cap = 0;
-    HaloRec *d_halo_s = nullptr;   /* painted halos in spatial order (tiled painters) */
-    TileHalo *d_hot = nullptr;     /* the same, packed for the FP32 painter */
+    /* painted halos in spatial order (tiled painters) */
+    BinHalo *d_bin = nullptr;      /* fields the binning reads */
+    TileHalo *d_hot = nullptr;     /* fields the FP32 painter stages (FP32 mode) */
+    HaloRec *d_halo_s = nullptr;   /* full records (FP64 tiled mode) */
     uint32_t *d_keys = nullptr, *d_vals = nullptr;
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -157,15 +165,23 @@ static int build_tiles(p2m_ctx *c, int log2w)
         acc += (npr + W - 1) / W;
     }
     c->n_tiles = acc;
-    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off);
+    std::vector<int32_t> tile_ring((size_t)acc);
+    for (int r = 1; r <= c->n_rings; ++r) {
+        const int64_t e = r < c->n_rings ? c->tile_base[r] : acc;
+        for (int64_t t = c->tile_base[r - 1]; t < e; ++t) tile_ring[(size_t)t] = r;
+    }
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_tile_ring);
     c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_fill = nullptr; c->d_tile_off = nullptr;
+    c->d_tile_ring = nullptr;
     int rc = 0;
+    rc |= dmalloc((void **)&c->d_tile_ring, sizeof(int32_t) * acc);
     rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
     rc |= dmalloc((void **)&c->d_tile_cnt, sizeof(uint32_t) * (c->n_tiles + 1));
     rc |= dmalloc((void **)&c->d_tile_fill, sizeof(uint32_t) * (c->n_tiles + 1));
     rc |= dmalloc((void **)&c->d_tile_off, sizeof(unsigned long long) * (c->n_tiles + 1));
     if (rc) return P2M_E_NOMEM;
     HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_tile_ring, tile_ring.data(), sizeof(int32_t) * acc, hipMemcpyHostToDevice));
     c->scan_tmp_bytes = 0;
     c->log2w = log2w;
     return 0;
@@ -253,8 +269,9 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
+    dfree(c->d_tile_ring); dfree(c->d_group); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
-    dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
+    dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
     for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -416,7 +433,8 @@ int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
     if (c->cfg.precision == P2M_PREC_FP32_COMP && tile_wg > 256) return P2M_E_INVALID;
     int l = 0;
     while ((1 << l) < tile_w) ++l;
-    if (p2m_tile_lds_bytes(c->cfg.precision == P2M_PREC_FP64 ? 1 : 2, c->cfg.n_maps, l, c->d_count != nullptr) > 160 * 1024)
+    if (p2m_tile_lds_bytes(c->cfg.precision == P2M_PREC_FP64 ? 1 : 2, c->cfg.n_maps, l, c->d_count != nullptr,
+                           c->have_table[0] ? c->g.nrt : 200) > 160 * 1024)
         return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
     int rc = build_tiles(c, l);
@@ -469,6 +487,7 @@ static int ensure_halo_cap(p2m_ctx *c, int64_t n)
     dfree(c->d_ring_off); c->d_ring_off = nullptr;
     dfree(c->d_halo_s); c->d_halo_s = nullptr;
     dfree(c->d_hot); c->d_hot = nullptr;
+    dfree(c->d_bin); c->d_bin = nullptr;
     dfree(c->d_keys); c->d_keys = nullptr;
     dfree(c->d_vals); c->d_vals = nullptr;
     int64_t cap = std::max<int64_t>(n, 1024);
@@ -477,8 +496,12 @@ static int ensure_halo_cap(p2m_ctx *c, int64_t n)
     if (dmalloc((void **)&c->d_nr, sizeof(uint32_t) * (cap + 1))) return P2M_E_NOMEM;
     if (dmalloc((void **)&c->d_ring_off, sizeof(unsigned long long) * (cap + 1))) return P2M_E_NOMEM;
     if (c->cfg.precision != P2M_PREC_PLAIN_FP64 && c->cfg.nside > 0) {
-        if (dmalloc((void **)&c->d_halo_s, sizeof(HaloRec) * cap)) return P2M_E_NOMEM;
-        if (dmalloc((void **)&c->d_hot, sizeof(TileHalo) * cap)) return P2M_E_NOMEM;
+        if (dmalloc((void **)&c->d_bin, sizeof(BinHalo) * cap)) return P2M_E_NOMEM;
+        if (c->cfg.precision == P2M_PREC_FP32_COMP) {
+            if (dmalloc((void **)&c->d_hot, sizeof(TileHalo) * cap)) return P2M_E_NOMEM;
+        } else {
+            if (dmalloc((void **)&c->d_halo_s, sizeof(HaloRec) * cap)) return P2M_E_NOMEM;
+        }
         if (dmalloc((void **)&c->d_keys, sizeof(uint32_t) * 2 * cap)) return P2M_E_NOMEM;
         if (dmalloc((void **)&c->d_vals, sizeof(uint32_t) * 2 * cap)) return P2M_E_NOMEM;
     }
@@ -504,6 +527,12 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
     a.n_cand0 = c->d_ctr + Ctr::N_CAND0;
     a.n_flat_out = c->d_ctr + Ctr::N_FLAT;
     a.counters = c->d_ctr + Ctr::DEC;
+    a.keys = nullptr; a.vals = nullptr; a.nphi = 1; a.key_max = 0;
+    if (c->d_bin && c->cfg.nside > 0) {
+        a.keys = c->d_keys; a.vals = c->d_vals;
+        a.nphi = std::max(1, c->cfg.nside / 128);
+        a.key_max = p2m_key_max(c->cfg.nside, a.nphi);
+    }
     p2m_launch_decode(a, c->stream);
     return hipGetLastError() == hipSuccess ? 0 : P2M_E_HIP;
 }
@@ -543,11 +572,12 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
             if (allow_tune) { int rc = autotune(c, n_halo, n_flat); if (rc) return rc; }
             else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
         }
-        /* painted halos in spatial order: neighbouring halos share tiles */
+        /* painted halos in spatial order (keys written by decode):
+         * neighbouring halos share tiles */
         {
-            const int32_t nphi = std::max(1, (4 * c->cfg.nside) >> c->log2w);
+            const uint32_t key_max = p2m_key_max(c->cfg.nside, std::max(1, c->cfg.nside / 128));
             size_t need = 0;
-            if (p2m_sort_halos(c->d_halo, c->d_halo_s, c->d_hot, n_halo, c->cfg.nside, nphi, c->g.nrt, c->g.nmht,
+            if (p2m_sort_halos(c->d_halo, n_halo, key_max, c->d_bin, c->d_hot, c->d_halo_s, c->g.nrt, c->g.nmht,
                                c->d_keys, c->d_vals, nullptr, &need, c->stream)) return P2M_E_HIP;
             if (need > c->sort_tmp_bytes || !c->d_sort_tmp) {
                 dfree(c->d_sort_tmp); c->d_sort_tmp = nullptr; c->sort_tmp_bytes = 0;
@@ -555,13 +585,13 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
                 c->sort_tmp_bytes = need + 256;
             }
             size_t bytes = c->sort_tmp_bytes;
-            if (p2m_sort_halos(c->d_halo, c->d_halo_s, c->d_hot, n_halo, c->cfg.nside, nphi, c->g.nrt, c->g.nmht,
+            if (p2m_sort_halos(c->d_halo, n_halo, key_max, c->d_bin, c->d_hot, c->d_halo_s, c->g.nrt, c->g.nmht,
                                c->d_keys, c->d_vals, c->d_sort_tmp, &bytes, c->stream)) return P2M_E_HIP;
         }
-        const HaloRec *hs = c->d_halo_s;
+        const uint32_t *order = c->d_vals + n_halo;   /* spatial position -> slot */
         BinArgs b;
         memset(&b, 0, sizeof(b));
-        b.halos = hs; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
+        b.halos = c->d_bin; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
         b.nr = c->d_nr; b.ring_off = c->d_ring_off;
         b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt; b.tile_fill = c->d_tile_fill;
         b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc;
@@ -577,7 +607,13 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
             if (dmalloc((void **)&c->d_runs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
             c->runs_cap = cap;
         }
-        b.n_items = n_items; b.runs = c->d_runs;
+        if ((int64_t)(n_items / 64 + 2) > c->group_cap || !c->d_group) {
+            dfree(c->d_group); c->d_group = nullptr; c->group_cap = 0;
+            const int64_t cap = (int64_t)((n_items / 64 + 2) * 1.25) + 1024;
+            if (dmalloc((void **)&c->d_group, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
+            c->group_cap = cap;
+        }
+        b.n_items = n_items; b.runs = c->d_runs; b.group_halo = c->d_group;
         p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_fill_u32(c->d_tile_fill, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_seg_count(b, c->stream);
@@ -610,9 +646,9 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         HIPCHK(hipEventRecord(c->ev[3], c->stream));
         TileArgs t;
         memset(&t, 0, sizeof(t));
-        t.halos = hs; t.hot = c->d_hot; t.nside = c->cfg.nside; t.log2w = c->log2w; t.n_rings = c->n_rings;
+        t.halos = c->d_halo_s; t.hot = c->d_hot; t.nside = c->cfg.nside; t.log2w = c->log2w; t.n_rings = c->n_rings;
         t.n_maps = c->cfg.n_maps; t.profile = c->cfg.profile; t.accumulate = c->hp_zero ? 0 : 1;
-        t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
+        t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.tile_ring = c->d_tile_ring; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
         t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
         t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
         t.band_c = 4e-6f;
@@ -630,7 +666,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         if (mode == P2M_PREC_FP32_COMP) {
             FallbackArgs f;
             memset(&f, 0, sizeof(f));
-            f.halos = hs; f.fb = c->d_fb; f.fb_count = pc + 6; f.fb_cap = c->fb_cap;
+            f.halos = c->d_halo; f.order = order; f.fb = c->d_fb; f.fb_count = pc + 6; f.fb_cap = c->fb_cap;
             f.nside = c->cfg.nside; f.n_maps = c->cfg.n_maps; f.profile = c->cfg.profile;
             f.g = c->g; f.n_flat = c->n_flat;
             for (int m = 0; m < c->cfg.n_maps; ++m) { f.tab_flat[m] = c->d_flat[m]; f.map32[m] = c->d_hp32[m]; }
@@ -641,7 +677,19 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         HIPCHK(hipEventRecord(c->ev[3], c->stream));
     }
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
-    if (c->cfg.npix_fs > 0) p2m_launch_paint_fs(pa, 0, c->stream);
+    if (c->cfg.npix_fs > 0 && n_flat > 0) {
+        if (n_flat + 1 > c->fs_cap || !c->d_fs_cnt) {
+            dfree(c->d_fs_cnt); dfree(c->d_fs_off); c->d_fs_cnt = nullptr; c->d_fs_off = nullptr; c->fs_cap = 0;
+            const int64_t cap = n_flat + 1 + 1024;
+            if (dmalloc((void **)&c->d_fs_cnt, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
+            if (dmalloc((void **)&c->d_fs_off, sizeof(unsigned long long) * cap)) return P2M_E_NOMEM;
+            c->fs_cap = cap;
+        }
+        pa.fs_cnt = c->d_fs_cnt; pa.fs_off = c->d_fs_off;
+        p2m_launch_fs_counts(pa, c->stream);
+        { int rc = scan_counts(c, c->d_fs_cnt, c->d_fs_off, n_flat + 1); if (rc) return rc; }
+        p2m_launch_fs_paint(pa, c->stream);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     c->hp_zero = false;
@@ -832,7 +880,7 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat)
     }
     float best_ms = 1e30f;
     for (auto &cd : cand) {
-        if (p2m_tile_lds_bytes(mode_ld, c->cfg.n_maps, cd[0], c->d_count != nullptr) > 160 * 1024) continue;
+        if (p2m_tile_lds_bytes(mode_ld, c->cfg.n_maps, cd[0], c->d_count != nullptr, c->g.nrt) > 160 * 1024) continue;
         int rc = build_tiles(c, cd[0]); if (rc) return rc;
         c->wg = cd[1]; c->tuned = true;
         c->hp_zero = true;

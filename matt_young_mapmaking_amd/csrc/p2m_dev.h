@@ -65,6 +65,13 @@ struct HaloRec {
     int32_t jmin, jmax, kmin, kmax;  /* flat-sky index box (haloproject.f90:124-128) */
 };
 
+/* The fields of a painted halo that the binning reads (query_disc per
+ * ring), in spatial order. */
+struct BinHalo {
+    double cosang, zz0, a, phi0;
+    int32_t irmin, irmax;
+};
+
 /* The fields of a painted halo that the FP32 tile painter stages per
  * segment, packed into one 64-B line (written in spatial order). */
 struct __attribute__((aligned(64))) TileHalo {
@@ -244,7 +251,8 @@ P2M_DEV double angdist(double ax, double ay, double az, double bx, double by, do
 
 /* phi half-width of the disc on a ring at height z (discphirange_at_z,
  * pix_tools.F90:712-760); -1000 = no intersection. */
-P2M_DEV double disc_dphi(const HaloRec &h, double z)
+template <typename H>
+P2M_DEV double disc_dphi(const H &h, double z)
 {
     double dphi = -1000.0;
     double b = h.cosang - z * h.zz0;
@@ -260,11 +268,23 @@ P2M_DEV double disc_dphi(const HaloRec &h, double z)
     return dphi;
 }
 
+/* x mod n in [0, n) (modulo of pixels_on_edge); the disc edges lie within a
+ * ring's length or two of zero, so the 64-bit division is rarely needed */
+P2M_DEV int64_t mod_ring(int64_t x, int64_t n)
+{
+    if (x >= 0 && x < n) return x;
+    if (x < 0 && x >= -n) return x + n;
+    if (x >= n && x < 2 * n) return x - n;
+    const int64_t r = x % n;
+    return r < 0 ? r + n : r;
+}
+
 /* Pixel range of the disc on ring `ring` (pixels_on_edge, pix_tools.F90:762-800,
  * and discedge2fulldisc, pixel_routines.F90:1823-1879).  Returns the number
  * of pixels np (0 if none) and the first in-ring index lo (pixels lo..lo+np-1
  * modulo npr). */
-P2M_DEV int32_t disc_ring_run(const HaloRec &h, int32_t nside, int32_t ring, int32_t npr, int32_t kshift, int32_t &lo)
+template <typename H>
+P2M_DEV int32_t disc_ring_run(const H &h, int32_t nside, int32_t ring, int32_t npr, int32_t kshift, int32_t &lo)
 {
     double z = ring2z(nside, ring);
     double dphi = disc_dphi(h, z);
@@ -274,9 +294,9 @@ P2M_DEV int32_t disc_ring_run(const HaloRec &h, int32_t nside, int32_t ring, int
     int64_t il = (int64_t)p2m_ceil(((double)npr * (h.phi0 - dphi)) / HPX_TWOPI - shift);
     int64_t ih = (int64_t)p2m_floor(((double)npr * (h.phi0 + dphi)) / HPX_TWOPI - shift);
     if (ih < il) return 0;
-    int64_t l = il % npr; if (l < 0) l += npr;
-    int64_t hh = ih % npr; if (hh < 0) hh += npr;
-    int64_t np = (hh - l) % npr; if (np < 0) np += npr;
+    const int64_t l = mod_ring(il, npr);
+    const int64_t hh = mod_ring(ih, npr);
+    int64_t np = mod_ring(hh - l, npr);
     np += 1;
     if (np > npr) np = npr;
     lo = (int32_t)l;

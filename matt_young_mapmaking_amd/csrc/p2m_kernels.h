@@ -33,6 +33,9 @@ struct DecodeArgs {
     int64_t cap;
     uint32_t *flat_idx;          /* slots of halos the flat-sky painter covers */
     unsigned long long *n_flat_out;
+    uint32_t *keys, *vals;       /* spatial sort key and slot of each record (tiled painters) */
+    int32_t nphi;                /* azimuth bins of the key */
+    uint32_t key_max;            /* key of inert records (sorted last) */
     unsigned long long *counters;/* sharded: [0] passed radial cut, [2] radius errors */
 };
 
@@ -42,6 +45,8 @@ struct PaintArgs {
     const uint32_t *flat_idx;     /* flat-sky halo list (tiled painters) */
     const unsigned long long *n_flat_dev;
     int64_t n_flat_host;          /* its length, known on the host */
+    uint32_t *fs_cnt;             /* box pixels per listed halo (n_flat_host + 1) */
+    unsigned long long *fs_off;   /* their exclusive scan */
     int32_t nside, npix_fs, n_maps, profile;
     double fov, dpix;
     TabGeom g;
@@ -57,12 +62,13 @@ struct PaintArgs {
 };
 
 struct BinArgs {
-    const HaloRec *halos;
+    const BinHalo *halos;         /* in spatial order */
     int64_t n_halo;
     int32_t nside, log2w;
     uint32_t *nr;                 /* rings per halo (n_halo + 1 entries, last 0) */
     const unsigned long long *ring_off;  /* exclusive scan of nr */
     unsigned long long n_items;   /* total (halo, ring) items */
+    uint32_t *group_halo;         /* halo of item 64 g (n_items / 64 + 1 entries) */
     uint2 *runs;                  /* per item: (first in-ring pixel, pixel count) */
     const int32_t *tile_base;     /* first tile of ring r+1 */
     uint32_t *tile_cnt;
@@ -78,6 +84,7 @@ struct TileArgs {
     int32_t nside, log2w, n_rings, n_maps, profile, accumulate;
     int64_t n_tiles;
     const int32_t *tile_base;
+    const int32_t *tile_ring;     /* ring of each tile */
     const unsigned long long *tile_off;
     const uint2 *segs;
     TabGeom g;
@@ -99,7 +106,8 @@ struct TileArgs {
 };
 
 struct FallbackArgs {
-    const HaloRec *halos;
+    const HaloRec *halos;         /* records in slot order */
+    const uint32_t *order;        /* spatial position -> slot */
     const uint4 *fb;
     const unsigned long long *fb_count;
     int64_t fb_cap;
@@ -121,14 +129,17 @@ void p2m_launch_clear_ctr(unsigned long long *ctr, int first, int count, hipStre
 void p2m_launch_decode(const DecodeArgs &a, hipStream_t s);
 void p2m_launch_paint_plain_hp(const PaintArgs &a, hipStream_t s);
 void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s);
+void p2m_launch_fs_counts(const PaintArgs &a, hipStream_t s);
+void p2m_launch_fs_paint(const PaintArgs &a, hipStream_t s);
 void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
-int p2m_sort_halos(const HaloRec *in, HaloRec *out, TileHalo *hot, int64_t n, int32_t nside, int32_t nphi,
+uint32_t p2m_key_max(int32_t nside, int32_t nphi);
+int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin, TileHalo *hot, HaloRec *full,
                    int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
                    hipStream_t s);
 int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
-size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership);
+size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int nrt);
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);
 int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s);
 void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double mean, double *out,

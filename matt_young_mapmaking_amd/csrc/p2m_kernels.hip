@@ -266,7 +266,23 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
             }
         }
         const bool flat = k < kend && painted && h.flat;
-        if (k < kend && k < (unsigned long long)A.cap) A.out[k] = h;
+        if (k < kend && k < (unsigned long long)A.cap) {
+            A.out[k] = h;
+            if (A.keys) {
+                /* spatial key: band of 8 rings of the disc centre, then its azimuth bin */
+                uint32_t key = A.key_max;
+                if (painted && A.nside > 0) {
+                    const int32_t rc = ring_num(A.nside, h.zz0);
+                    double ph = h.phi0;
+                    if (ph < 0.0) ph += HPX_TWOPI;
+                    int32_t pb = (int32_t)(ph * (double)A.nphi / HPX_TWOPI);
+                    pb = pb < 0 ? 0 : (pb >= A.nphi ? A.nphi - 1 : pb);
+                    key = (uint32_t)((rc - 1) >> 3) * (uint32_t)A.nphi + (uint32_t)pb;
+                }
+                A.keys[k] = key;
+                A.vals[k] = (uint32_t)k;
+            }
+        }
         const unsigned long long fs = wave_append(A.n_flat_out, flat);
         if (flat) A.flat_idx[fs] = (uint32_t)k;
     }
@@ -349,12 +365,10 @@ void p2m_launch_paint_plain_hp(const PaintArgs &a, hipStream_t s)
 
 __global__ void __launch_bounds__(256) k_paint_fs(PaintArgs A, int plain)
 {
-    int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    int64_t hidx = blockIdx.x * 4LL + wave;
     unsigned long long n_upd = 0;
     if (plain) {
         /* baseline: one work item per halo */
-        hidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+        const int64_t hidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
         if (hidx < A.n_halo) {
             const HaloRec &hr = A.halos[hidx];
             if (hr.flat) {
@@ -383,27 +397,67 @@ __global__ void __launch_bounds__(256) k_paint_fs(PaintArgs A, int plain)
         wave_count(&A.counters[2], n_upd);
         return;
     }
-    const unsigned long long nfl = *A.n_flat_dev;
-    if ((unsigned long long)hidx < nfl) {
-        const HaloRec &h = A.halos[A.flat_idx[hidx]];
-        int nj = h.jmax - h.jmin + 1, nk = h.kmax - h.kmin + 1;
-        int64_t tot = (int64_t)nj * nk;
-        double hf = A.fov / 2.0;
-        for (int64_t q = lane; q < tot; q += 64) {
-            int k = h.kmin + (int)(q / nj), j = h.jmin + (int)(q % nj);
-            double phip = -hf + ((double)k - 0.5) * A.dpix;
-            double thetap = -hf + ((double)j - 0.5) * A.dpix;
-            double dphi = phip - h.ph, dth = thetap - h.th;
-            double angle = p2m_sqrt(dphi * dphi + dth * dth);
-            if (angle < h.dtheta) {
-                int64_t pix = (int64_t)(j - 1) + (int64_t)A.npix_fs * (k - 1);
-                n_upd++;
-                for (int m = 0; m < A.n_maps; ++m)
-                    atomicAdd(&A.fs64[m][pix], h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle));
-                if (A.fs_count) {
-                    atomicAdd(&A.fs_count[pix], 1u);
-                    atomicAdd(&A.fs_hash[pix], (unsigned long long)halo_hash(h.cat));
-                }
+}
+
+/* box pixels of each listed flat-sky halo (haloproject.f90:124-135) */
+__global__ void __launch_bounds__(256) k_fs_counts(PaintArgs A)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= A.n_flat_host; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t n = 0;
+        if (i < A.n_flat_host) {
+            const HaloRec &h = A.halos[A.flat_idx[i]];
+            n = (uint32_t)((h.jmax - h.jmin + 1) * (h.kmax - h.kmin + 1));
+        }
+        A.fs_cnt[i] = n;
+    }
+}
+
+/* Flat-sky painter over the flattened (halo, box pixel) space: every wave
+ * takes 64 consecutive box pixels, so a few large boxes do not serialise
+ * on one wave.  FP64 angle and value (haloproject.f90:130-150), global
+ * FP64 atomics into the map. */
+__global__ void __launch_bounds__(256) k_fs_paint(PaintArgs A)
+{
+    const int lane = __lane_id();
+    const int64_t nfl = A.n_flat_host;
+    const unsigned long long total = A.fs_off[nfl];
+    const unsigned long long ngroup = (total + 63) >> 6;
+    const unsigned long long nw = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
+    const double hf = A.fov / 2.0;
+    unsigned long long n_upd = 0;
+    for (unsigned long long g = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) >> 6; g < ngroup; g += nw) {
+        const unsigned long long c = g << 6;
+        int64_t lo = 0, hi = nfl - 1;           /* halo holding pixel c (wave-uniform) */
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (A.fs_off[mid] <= c) lo = mid; else hi = mid - 1;
+        }
+        const unsigned long long item = c + lane;
+        const int64_t hl = lo + lane;
+        const unsigned long long my_off = hl <= nfl ? A.fs_off[hl] : ~0ULL;
+        int jj = 0;
+        for (int step = 32; step > 0; step >>= 1) {
+            const unsigned long long o = __shfl(my_off, jj + step);
+            if (o <= item) jj += step;
+        }
+        const unsigned long long hoff = __shfl(my_off, jj);
+        if (item >= total) continue;
+        const HaloRec &h = A.halos[A.flat_idx[lo + jj]];
+        const int nj = h.jmax - h.jmin + 1;
+        const int q = (int)(item - hoff);
+        const int k = h.kmin + q / nj, j = h.jmin + q % nj;
+        const double phip = -hf + ((double)k - 0.5) * A.dpix;
+        const double thetap = -hf + ((double)j - 0.5) * A.dpix;
+        const double dphi = phip - h.ph, dth = thetap - h.th;
+        const double angle = p2m_sqrt(dphi * dphi + dth * dth);
+        if (angle < h.dtheta) {
+            const int64_t pix = (int64_t)(j - 1) + (int64_t)A.npix_fs * (k - 1);
+            n_upd++;
+            for (int m = 0; m < A.n_maps; ++m)
+                atomicAdd(&A.fs64[m][pix], h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle));
+            if (A.fs_count) {
+                atomicAdd(&A.fs_count[pix], 1u);
+                atomicAdd(&A.fs_hash[pix], (unsigned long long)halo_hash(h.cat));
             }
         }
     }
@@ -414,50 +468,25 @@ void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s)
 {
     if (a.n_halo <= 0 || a.npix_fs <= 0) return;
     if (plain) hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_halo + 255) / 256)), dim3(256), 0, s, a, 1);
-    else if (a.n_flat_host > 0) hipLaunchKernelGGL(k_paint_fs, dim3((unsigned)((a.n_flat_host + 3) / 4)), dim3(256), 0, s, a, 0);
+}
+
+void p2m_launch_fs_counts(const PaintArgs &a, hipStream_t s)
+{
+    if (a.n_flat_host <= 0 || a.npix_fs <= 0) return;
+    int64_t g = (a.n_flat_host + 1 + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_fs_counts, dim3((unsigned)g), dim3(256), 0, s, a);
+}
+
+void p2m_launch_fs_paint(const PaintArgs &a, hipStream_t s)
+{
+    if (a.n_flat_host <= 0 || a.npix_fs <= 0) return;
+    hipLaunchKernelGGL(k_fs_paint, dim3(2048), dim3(256), 0, s, a);
 }
 
 /* ===================================================================== */
 /* binning of (halo, ring) runs into ring tiles                           */
 /* ===================================================================== */
-
-/* The binning works on the flattened (halo, ring) item space: item i
- * belongs to halo h with ring_off[h] <= i < ring_off[h+1] and is ring
- * irmin(h) + i - ring_off[h].  Every halo has at least one ring, so the 64
- * consecutive items a wave handles span at most 64 halos; each lane finds
- * its halo with a 6-step shuffle search over off[h0 .. h0+63]. */
-template <typename F>
-__device__ static inline void wave_items(const unsigned long long *off, int64_t n_halo, unsigned long long total, F f)
-{
-    const int lane = __lane_id();
-    const unsigned long long wid = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) >> 6;
-    const unsigned long long nw = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
-    const unsigned long long per = (total + nw - 1) / nw;
-    const unsigned long long start = wid * per;
-    unsigned long long end = start + per;
-    if (end > total) end = total;
-    if (start >= end) return;
-    int64_t lo = 0, hi = n_halo - 1;
-    while (lo < hi) {
-        int64_t mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= start) lo = mid; else hi = mid - 1;
-    }
-    int64_t h = lo;
-    for (unsigned long long c = start; c < end; c += 64) {
-        while (h + 1 < n_halo && off[h + 1] <= c) ++h;
-        const unsigned long long item = c + lane;
-        const int64_t hl = h + lane;
-        const unsigned long long my_off = hl <= n_halo ? off[hl] : ~0ULL;
-        int j = 0;
-        for (int step = 32; step > 0; step >>= 1) {
-            unsigned long long o = __shfl(my_off, j + step);
-            if (o <= item) j += step;
-        }
-        const unsigned long long hoff = __shfl(my_off, j);
-        if (item < end) f(item, h + j, item - hoff);
-        h += __shfl(j, 63);
-    }
-}
 
 /* tile pieces of one ring run [lo, lo+np) (wrapping at npr): f(tile, lo_in_tile, len) */
 template <typename F>
@@ -483,7 +512,7 @@ __global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t nr = 0;
         if (i < B.n_halo) {
-            const HaloRec &h = B.halos[i];
+            const BinHalo &h = B.halos[i];
             nr = h.irmax >= h.irmin ? (uint32_t)(h.irmax - h.irmin + 1) : 1u;
         }
         B.nr[i] = nr;
@@ -530,23 +559,18 @@ __device__ static inline void bin_reset(BinHash *L)
     if (threadIdx.x == 0) L->npieces = 0;
 }
 
-/* the items [start, end) of one wave, with their halo and ring offset:
- * f(item, halo, k) for each item of this lane (items of a 64-group share
- * at most 64 halos: every halo has at least one ring) */
+/* the items [start, end) of one wave (start a multiple of 64), with their
+ * halo and ring offset: f(item, halo, k) for each item of this lane.  The
+ * halo holding item 64g is group_halo[g]; the 64 items of a group span at
+ * most 64 halos (every halo has at least one ring), so each lane finds its
+ * halo by a 6-step shuffle search over off[h .. h+63]. */
 template <typename F>
-__device__ static inline void wave_range(const unsigned long long *off, int64_t n_halo, unsigned long long start,
-                                         unsigned long long end, F f)
+__device__ static inline void wave_range(const unsigned long long *off, const uint32_t *group_halo, int64_t n_halo,
+                                         unsigned long long start, unsigned long long end, F f)
 {
-    if (start >= end) return;
     const int lane = __lane_id();
-    int64_t lo = 0, hi = n_halo - 1;
-    while (lo < hi) {
-        int64_t mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= start) lo = mid; else hi = mid - 1;
-    }
-    int64_t h = lo;
     for (unsigned long long c = start; c < end; c += 64) {
-        while (h + 1 < n_halo && off[h + 1] <= c) ++h;
+        const int64_t h = group_halo[c >> 6];
         const unsigned long long item = c + lane;
         const int64_t hl = h + lane;
         const unsigned long long my_off = hl <= n_halo ? off[hl] : ~0ULL;
@@ -557,7 +581,15 @@ __device__ static inline void wave_range(const unsigned long long *off, int64_t 
         }
         const unsigned long long hoff = __shfl(my_off, j);
         if (item < end) f(item, h + j, item - hoff);
-        h += __shfl(j, 63);
+    }
+}
+
+/* group_halo[g] = the halo holding item 64 g */
+__global__ void __launch_bounds__(256) k_group_heads(BinArgs B)
+{
+    for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < B.n_halo; h += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long g0 = (B.ring_off[h] + 63) >> 6, g1 = (B.ring_off[h + 1] + 63) >> 6;
+        for (unsigned long long g = g0; g < g1; ++g) B.group_halo[g] = (uint32_t)h;
     }
 }
 
@@ -575,8 +607,8 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
     for (unsigned long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
         unsigned long long a = ch * BIN_CH + wv * per, e = a + per;
         if (e > B.n_items) e = B.n_items;
-        wave_range(B.ring_off, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
-            const HaloRec &h = B.halos[hi];
+        wave_range(B.ring_off, B.group_halo, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+            const BinHalo &h = B.halos[hi];
             const int32_t ring = h.irmin + (int32_t)k;
             int32_t lo = 0, np = 0;
             int32_t npr, kshift; int64_t first;
@@ -606,24 +638,11 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
     block_count(&B.counters[4], n_seg, lds16);
 }
 
-/* empty rings between non-empty rings of one disc, counted once per gap
- * (the row misalignment of discedge2fulldisc, quirk Q15) */
-__global__ void __launch_bounds__(256) k_ring_gaps(BinArgs B)
-{
-    __shared__ unsigned long long lds16[16];
-    unsigned long long n_gaps = 0;
-    wave_items(B.ring_off, B.n_halo, B.n_items, [&](unsigned long long item, int64_t hi, unsigned long long k) {
-        if (k == 0 || B.runs[item].y != 0 || B.runs[item - 1].y == 0) return;
-        const unsigned long long endi = B.ring_off[hi + 1];
-        for (unsigned long long j = item + 1; j < endi; ++j)
-            if (B.runs[j].y != 0) { n_gaps++; break; }
-    });
-    block_count(&B.counters[3], n_gaps, lds16);
-}
-
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
     __shared__ BinLDS L;
+    __shared__ unsigned long long lds16[16];
+    unsigned long long n_gaps = 0;
     const unsigned long long nchunk = (B.n_items + BIN_CH - 1) / BIN_CH;
     bin_reset(&L);
     __syncthreads();
@@ -632,10 +651,20 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
     for (unsigned long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
         unsigned long long a = ch * BIN_CH + wv * per, e = a + per;
         if (e > B.n_items) e = B.n_items;
-        wave_range(B.ring_off, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+        wave_range(B.ring_off, B.group_halo, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
             const uint2 run = B.runs[item];
-            if (run.y == 0) return;
-            const HaloRec &h = B.halos[hi];
+            if (run.y == 0) {
+                /* an empty ring between non-empty rings of one disc, counted
+                 * once per gap (the row misalignment of discedge2fulldisc,
+                 * quirk Q15) */
+                if (k > 0 && B.runs[item - 1].y != 0) {
+                    const unsigned long long endi = B.ring_off[hi + 1];
+                    for (unsigned long long j = item + 1; j < endi; ++j)
+                        if (B.runs[j].y != 0) { n_gaps++; break; }
+                }
+                return;
+            }
+            const BinHalo &h = B.halos[hi];
             const int32_t ring = h.irmin + (int32_t)k;
             int32_t npr, kshift; int64_t first;
             ring_info(B.nside, ring, npr, kshift, first);
@@ -677,59 +706,50 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
         bin_reset(&L);
         __syncthreads();
     }
+    block_count(&B.counters[3], n_gaps, lds16);
 }
 
 /* ------------------------------------------------------------------ */
 /* spatial order of the painted halos                                  */
 /* ------------------------------------------------------------------ */
 
-/* key = (band of 8 rings of the disc centre) * nphi + (azimuth bin of the
- * centre): halos with equal or adjacent keys share ring tiles.  Inert
- * records (no rings) sort last. */
-__global__ void __launch_bounds__(256) k_halo_keys(const HaloRec *h, int64_t n, int32_t nside, int32_t nphi,
-                                                   uint32_t key_max, uint32_t *keys, uint32_t *vals)
+__global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const uint32_t *order, int64_t n,
+                                                      BinHalo *bin, TileHalo *hot, HaloRec *full, int32_t nrt, int32_t nmht)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const HaloRec &r = h[i];
-        uint32_t key = key_max;
-        if (r.irmax >= r.irmin) {
-            const int32_t rc = ring_num(nside, r.zz0);
-            double ph = r.phi0;
-            if (ph < 0.0) ph += HPX_TWOPI;
-            int32_t pb = (int32_t)(ph * (double)nphi / HPX_TWOPI);
-            pb = pb < 0 ? 0 : (pb >= nphi ? nphi - 1 : pb);
-            key = (uint32_t)((rc - 1) >> 3) * (uint32_t)nphi + (uint32_t)pb;
+        const HaloRec &h = in[order[i]];
+        BinHalo b;
+        b.cosang = h.cosang; b.zz0 = h.zz0; b.a = h.a; b.phi0 = h.phi0;
+        b.irmin = h.irmin; b.irmax = h.irmax;
+        bin[i] = b;
+        if (hot) {
+            TileHalo t;
+            t.zz0 = h.zz0; t.s0 = h.s0; t.phi0 = h.phi0; t.thr = h.thr;
+            t.chi2 = (float)(h.chi_r * h.chi_r);
+            t.vh = h.chi_out ? 0.0f : (float)h.vh;
+            t.fc = (float)h.fc; t.fm = (float)h.fm;
+            t.colbase = nrt * ((h.imh - 1) + nmht * (h.ichi - 1));
+            t.pad = 0;
+            t.hash = halo_hash(h.cat);
+            hot[i] = t;
         }
-        keys[i] = key;
-        vals[i] = (uint32_t)i;
+        if (full) full[i] = h;
     }
 }
 
-__global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const uint32_t *order, int64_t n, HaloRec *out,
-                                                      TileHalo *hot, int32_t nrt, int32_t nmht)
+uint32_t p2m_key_max(int32_t nside, int32_t nphi)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const HaloRec h = in[order[i]];
-        out[i] = h;
-        TileHalo t;
-        t.zz0 = h.zz0; t.s0 = h.s0; t.phi0 = h.phi0; t.thr = h.thr;
-        t.chi2 = (float)(h.chi_r * h.chi_r);
-        t.vh = h.chi_out ? 0.0f : (float)h.vh;
-        t.fc = (float)h.fc; t.fm = (float)h.fm;
-        t.colbase = nrt * ((h.imh - 1) + nmht * (h.ichi - 1));
-        t.pad = 0;
-        t.hash = halo_hash(h.cat);
-        hot[i] = t;
-    }
+    const uint32_t nband = (uint32_t)((4 * nside - 2) >> 3) + 1;
+    return nband * (uint32_t)nphi;
 }
 
-int p2m_sort_halos(const HaloRec *in, HaloRec *out, TileHalo *hot, int64_t n, int32_t nside, int32_t nphi,
+/* Sorts the records by the key k_setup wrote (stable radix sort of
+ * (key, slot)), then writes the binning, FP32 and (if asked) full records
+ * in that order.  keys and vals hold 2n entries: input then output. */
+int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin, TileHalo *hot, HaloRec *full,
                    int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
                    hipStream_t s)
 {
-    /* keys and vals hold 2n entries each: input then output halves */
-    const uint32_t nband = (uint32_t)((4 * nside - 2) >> 3) + 1;
-    const uint32_t key_max = nband * (uint32_t)nphi;
     int bits = 1;
     while ((1u << bits) <= key_max && bits < 32) ++bits;
     if (!tmp) {
@@ -738,12 +758,11 @@ int p2m_sort_halos(const HaloRec *in, HaloRec *out, TileHalo *hot, int64_t n, in
         return e == hipSuccess ? 0 : -1;
     }
     if (n <= 0) return 0;
-    unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_halo_keys, dim3(g), dim3(256), 0, s, in, n, nside, nphi, key_max, keys, vals);
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, keys, keys + n, vals, vals + n,
                                                       (int)n, 0, bits, s);
     if (e != hipSuccess) return -1;
-    hipLaunchKernelGGL(k_gather_halos, dim3(g), dim3(256), 0, s, in, vals + n, n, out, hot, nrt, nmht);
+    unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_gather_halos, dim3(g), dim3(256), 0, s, in, vals + n, n, bin, hot, full, nrt, nmht);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -760,8 +779,8 @@ static unsigned chunk_grid(unsigned long long n_items)
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s)
 {
     if (b.n_items == 0) return;
+    hipLaunchKernelGGL(k_group_heads, dim3(bin_grid(b.n_halo)), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_seg_count, dim3(chunk_grid(b.n_items)), dim3(256), 0, s, b);
-    hipLaunchKernelGGL(k_ring_gaps, dim3(bin_grid((int64_t)b.n_items)), dim3(256), 0, s, b);
 }
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s)
 {
@@ -1052,19 +1071,22 @@ __global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
 /* --------------------------------------------------------------------- */
 
 #define FSEG 256
-#define FB_LDS 512
+#define FB_LDS 256
+#define PMAX 4096                  /* pixels per staged batch (>= the widest tile) */
 
 struct FastSeg {
+    double dphi0[FSEG];            /* phi offset of the ring's pixel 0 from the halo centre */
+    unsigned long long hh[FSEG];
+    uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
     uint32_t lo[FSEG];
     uint32_t pre[FSEG + 1];
     float A[FSEG], B[FSEG], thr_hi[FSEG], thr_lo[FSEG], chi2[FSEG], vh[FSEG];
     float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];
     int32_t colbase[FSEG];
-    double dphi0[FSEG];
-    unsigned long long hh[FSEG];
-    uint4 fb[FB_LDS];
-    uint32_t nfb;
+    uint32_t seg_of[PMAX / 4];     /* batch pixel -> segment (bytes) */
+    uint32_t wmax[16];
+    uint32_t nfb, nb;
 };
 
 /* sin^2(x), |x| <= pi/2, FP32 Taylor polynomials (to x^11 / x^10) */
@@ -1101,30 +1123,35 @@ __device__ static inline float log1p_small_rcp(float q)
     return 2.0f * __builtin_fmaf(s * s2, p, s);
 }
 
+/* byte k of a packed word */
+__device__ static inline uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xffu; }
+
 template <int NM>
 __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
 {
     extern __shared__ unsigned char smem_raw[];
     const int W = 1 << A.log2w;
-    float *accS = (float *)smem_raw;
-    float *accC = accS + NM * W;
+    /* LDS: acc (NM*W doubles) | hash (W u64) count (W u32) if membership | FastSeg | 1/r_node^2 (nrt+2 floats) */
+    double *acc = (double *)smem_raw;
     const size_t acc_bytes = (size_t)NM * W * 8;
-    uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes);
-    unsigned long long *lhash = (unsigned long long *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 4 : 0));
+    unsigned long long *lhash = (unsigned long long *)(smem_raw + acc_bytes);
+    uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 8 : 0));
     const size_t mem_bytes = A.count ? (size_t)W * 12 : 0;
-    FastSeg *sl = (FastSeg *)(smem_raw + acc_bytes + mem_bytes);
-    __shared__ uint32_t wsum[16];
+    FastSeg *sl = (FastSeg *)(smem_raw + ((acc_bytes + mem_bytes + 15) & ~(size_t)15));
+    float *invn = (float *)(sl + 1);
     __shared__ unsigned long long lds16[16];
     __shared__ unsigned long long fb_base;
+    __shared__ uint32_t wsum[16];
 
     const int nrt = A.g.nrt;
     const int64_t dmh = nrt, dch = (int64_t)nrt * A.g.nmht;
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6, nwv = blockDim.x >> 6;
     unsigned long long n_upd = 0, n_fb = 0;
-    /* deferred pixels go to this workgroup's shard of the list */
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
     unsigned long long *const fbc = A.fb_count + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE;
     uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
-    if (threadIdx.x == 0) sl->nfb = 0;
+    for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
+    if (tid == 0) sl->nfb = 0;
     __syncthreads();
 
     /* XCD-aware order: workgroup b runs on XCD b % 8; the workgroups of one
@@ -1135,13 +1162,8 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     const int64_t tbeg = xcd ? (int64_t)(blockIdx.x % 8) * A.n_tiles / 8 + blockIdx.x / 8 : blockIdx.x;
     const int64_t tend = xcd ? (int64_t)(blockIdx.x % 8 + 1) * A.n_tiles / 8 : A.n_tiles;
     for (int64_t t = tbeg; t < tend; t += tstep) {
-        int lo_r = 0, hi_r = A.n_rings - 1;
-        while (lo_r < hi_r) {
-            int mid = (lo_r + hi_r + 1) >> 1;
-            if (A.tile_base[mid] <= t) lo_r = mid; else hi_r = mid - 1;
-        }
-        const int32_t ring = lo_r + 1;
-        const int32_t col = (int32_t)(t - A.tile_base[lo_r]);
+        const int32_t ring = A.tile_ring[t];
+        const int32_t col = (int32_t)(t - A.tile_base[ring - 1]);
         int32_t npr, kshift; int64_t first;
         ring_info(A.nside, ring, npr, kshift, first);
         const int32_t ip0 = col * W;
@@ -1151,26 +1173,24 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
         ring_phi_param(A.nside, ring, off, den);
         const double kphi = HPX_HALFPI / den;
 
-        for (int q = threadIdx.x; q < width; q += blockDim.x) {
-            for (int m = 0; m < NM; ++m) { accS[m * W + q] = 0.0f; accC[m * W + q] = 0.0f; }
+        for (int q = tid; q < width; q += blockDim.x) {
+            for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
 
         const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
         for (uint64_t b0 = sbeg; b0 < send;) {
             __syncthreads();
-            int nb = (int)(send - b0 < FSEG ? send - b0 : FSEG);
-            const int cap = (int)blockDim.x < FSEG ? (int)blockDim.x : FSEG;
-            if (nb > cap) nb = cap;
+            int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
+            if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
-            const int tid = threadIdx.x;
-            if (tid < nb) {
-                uint2 sg = A.segs[b0 + tid];
+            if (tid < nstage) {
+                const uint2 sg = A.segs[b0 + tid];
                 const TileHalo &h = A.hot[sg.x];
                 sl->halo[tid] = sg.x;
                 sl->lo[tid] = sg.y >> 16;
                 len = sg.y & 0xffffu;
-                double dz = zr - h.zz0, ds = sth - h.s0;
+                const double dz = zr - h.zz0, ds = sth - h.s0;
                 sl->A[tid] = (float)(dz * dz + ds * ds);
                 sl->B[tid] = (float)((4.0 * sth) * h.s0);
                 sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
@@ -1186,100 +1206,112 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 sl->dphi0[tid] = off * kphi - h.phi0;
                 sl->hh[tid] = h.hash;
             }
+            /* inclusive scan of the staged lengths */
             uint32_t v = len;
-            const int lane = __lane_id();
             for (int o = 1; o < 64; o <<= 1) {
-                uint32_t u = __shfl_up(v, o);
+                const uint32_t u = __shfl_up(v, o);
                 if (lane >= o) v += u;
             }
-            const int wv = threadIdx.x >> 6;
             if (lane == 63) wsum[wv] = v;
-            __syncthreads();
+                    __syncthreads();
             uint32_t add = 0;
             for (int k = 0; k < wv; ++k) add += wsum[k];
-            if (tid < nb) sl->pre[tid + 1] = v + add;
+            v += add;
+            if (tid < nstage) sl->pre[tid + 1] = v;
             if (tid == 0) sl->pre[0] = 0;
             __syncthreads();
+            /* the batch: the leading segments whose pixels fit in PMAX (a
+             * segment is at most W <= PMAX pixels, so at least one) */
+            if (tid < nstage) {
+                const uint32_t e = sl->pre[tid + 1];
+                if (e <= PMAX && (tid + 1 == nstage || sl->pre[tid + 2] > PMAX)) sl->nb = (uint32_t)(tid + 1);
+            }
+            __syncthreads();
+            const int nb = (int)sl->nb;
             const uint32_t P = sl->pre[nb];
-            b0 += (uint64_t)nb;
-            for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-                int s_lo = 0, s_hi = nb - 1;
-                while (s_lo < s_hi) {
-                    int mid = (s_lo + s_hi + 1) >> 1;
-                    if (sl->pre[mid] <= p) s_lo = mid; else s_hi = mid - 1;
+            /* segment of every batch pixel, as bytes: each thread takes a run
+             * of whole words, finds the segment of its first pixel by binary
+             * search and walks forward */
+            {
+                const uint32_t nw = (P + 3) >> 2;
+                const uint32_t wpt = (nw + blockDim.x - 1) / blockDim.x;
+                const uint32_t w0 = tid * wpt;
+                const uint32_t w1 = w0 + wpt < nw ? w0 + wpt : nw;
+                if (w0 < w1) {
+                    const uint32_t p0 = w0 * 4;
+                    int lo_s = 0, hi_s = nb - 1;
+                    while (lo_s < hi_s) {
+                        const int mid = (lo_s + hi_s + 1) >> 1;
+                        if (sl->pre[mid] <= p0) lo_s = mid; else hi_s = mid - 1;
+                    }
+                    uint32_t sg = (uint32_t)lo_s, nxt = sl->pre[sg + 1];
+                    for (uint32_t w = w0; w < w1; ++w) {
+                        uint32_t o = 0;
+                        for (int bb = 0; bb < 4; ++bb) {
+                            const uint32_t p = w * 4 + bb;
+                            while (p >= nxt && (int)sg + 1 < nb) { ++sg; nxt = sl->pre[sg + 1]; }
+                            o |= sg << (8 * bb);
+                        }
+                        sl->seg_of[w] = o;
+                    }
                 }
-                const int s = s_lo;
+            }
+            b0 += (uint64_t)nb;
+            __syncthreads();
+            const unsigned char *segb = (const unsigned char *)sl->seg_of;
+            for (uint32_t p = tid; p < P; p += blockDim.x) {
+                const int s = segb[p];
                 const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
                 const int ip = ip0 + q;
                 double x = 0.5 * __builtin_fma((double)ip, kphi, sl->dphi0[s]);
                 if (x > HPX_HALFPI) x -= HPX_PI;
                 const float s2 = sin2f((float)x);
                 const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
-                if (c2 >= sl->thr_hi[s]) continue;
-                bool exact = c2 > sl->thr_lo[s];
-                float val[NM];
-                if (!exact) {
-                    const float h2 = 0.25f * c2;
-                    const float dd = 1.0f - 2.0f * h2;
-                    if (!(dd > 1e-3f)) {
-                        exact = true;
-                    } else {
-                        const float rd = __builtin_amdgcn_rcpf(dd);
-                        const float t2 = ((4.0f * h2) * (1.0f - h2)) * (rd * rd);
-                        const float r2 = sl->chi2[s] * t2;
-                        const float u = __builtin_amdgcn_logf(r2) * A.u_k1 - A.u_k0;
-                        const float fl = __builtin_floorf(u);
-                        const float fu = u - fl;
-                        if (u > A.u_max + A.band) {
-                            for (int m = 0; m < NM; ++m) val[m] = 0.0f;     /* r > rmaxt */
-                        } else if ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
-                                   (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))) {
-                            exact = true;
-                        } else {
-                            int ir;
-                            float fr;
-                            if (u < 0.0f) { ir = 1; fr = A.fr_clamp; }     /* r = rmint + 1e-5 */
-                            else {
-                                ir = (int)fl + 1;
-                                fr = 0.5f * log1p_small_rcp(__builtin_fmaf(r2, A.inv_node2f[ir - 1], -1.0f));
-                            }
-                            const int64_t c0 = (int64_t)sl->colbase[s] + (ir - 1);
-                            const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
-                            const float vh = sl->vh[s];
-                            for (int m = 0; m < NM; ++m) {
-                                const float *T = A.tab_prof[m];
-                                const float a0 = T[c0], a1 = T[c0 + 1];
-                                const float b0v = T[c0 + dmh], b1 = T[c0 + dmh + 1];
-                                const float cc0 = T[c0 + dch], cc1 = T[c0 + dch + 1];
-                                const float d0 = T[c0 + dmh + dch], d1 = T[c0 + dmh + dch + 1];
-                                float acc = w00 * __builtin_fmaf(fr, a1 - a0, a0);
-                                acc = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc);
-                                acc = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc);
-                                acc = __builtin_fmaf(w11, __builtin_fmaf(fr, d1 - d0, d0), acc);
-                                val[m] = vh * acc;
-                            }
-                        }
-                    }
-                }
-                if (exact) {
-                    uint32_t slot = atomicAdd(&sl->nfb, 1u);
-                    uint4 e = make_uint4((uint32_t)ring, (uint32_t)ip, sl->halo[s], 0u);
+                const float thr_hi = sl->thr_hi[s];
+                if (!(c2 < thr_hi)) continue;                      /* outside the disc */
+                const float h2 = 0.25f * c2;
+                const float dd = 1.0f - 2.0f * h2;
+                const float rd = __builtin_amdgcn_rcpf(dd);
+                const float t2 = ((4.0f * h2) * (1.0f - h2)) * (rd * rd);
+                const float r2 = sl->chi2[s] * t2;
+                const float u = __builtin_amdgcn_logf(r2) * A.u_k1 - A.u_k0;
+                const float fl = __builtin_floorf(u);
+                const float fu = u - fl;
+                const bool zero = u > A.u_max + A.band;           /* r > rmaxt: value 0 */
+                const bool defer = c2 > sl->thr_lo[s] || !(dd > 1e-3f) ||
+                                   (!zero && ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
+                                              (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))));
+                const bool clampr = u < 0.0f;                     /* r < rmint: r = rmint + 1e-5 */
+                int ir = clampr ? 1 : (int)fl + 1;
+                ir = ir < 1 ? 1 : (ir > nrt - 1 ? nrt - 1 : ir);
+                const float fr = clampr ? A.fr_clamp
+                                        : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
+                if (defer) {
+                    const uint32_t slot = atomicAdd(&sl->nfb, 1u);
+                    const uint4 e = make_uint4((uint32_t)ring, (uint32_t)ip, sl->halo[s], 0u);
                     if (slot < FB_LDS) sl->fb[slot] = e;
                     else {
-                        unsigned long long gs = atomicAdd(fbc, 1ULL);
+                        const unsigned long long gs = atomicAdd(fbc, 1ULL);
                         if (gs < (unsigned long long)fbs) fbr[gs] = e;
                     }
                     n_fb++;
                     continue;
                 }
+                const int64_t c0 = (int64_t)sl->colbase[s] + (ir - 1);
+                const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
+                const float vh = zero ? 0.0f : sl->vh[s];
                 n_upd++;
                 for (int m = 0; m < NM; ++m) {
-                    const float vv = val[m];
-                    const float old = atomicAdd(&accS[m * W + q], vv);
-                    const float sum = old + vv;
-                    const float bb = sum - old;
-                    const float err = (old - (sum - bb)) + (vv - bb);
-                    atomicAdd(&accC[m * W + q], err);
+                    const float *T = A.tab_prof[m];
+                    const float a0 = T[c0], a1 = T[c0 + 1];
+                    const float b0v = T[c0 + dmh], b1 = T[c0 + dmh + 1];
+                    const float cc0 = T[c0 + dch], cc1 = T[c0 + dch + 1];
+                    const float d0 = T[c0 + dmh + dch], d1 = T[c0 + dmh + dch + 1];
+                    float acc4 = w00 * __builtin_fmaf(fr, a1 - a0, a0);
+                    acc4 = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc4);
+                    acc4 = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc4);
+                    acc4 = __builtin_fmaf(w11, __builtin_fmaf(fr, d1 - d0, d0), acc4);
+                    atomicAdd(&acc[m * W + q], (double)(vh * acc4));
                 }
                 if (A.count) {
                     atomicAdd(&lcount[q], 1u);
@@ -1289,9 +1321,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
         }
         __syncthreads();
         const int64_t pix0 = first + ip0;
-        for (int q = threadIdx.x; q < width; q += blockDim.x) {
+        for (int q = tid; q < width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) {
-                double v = (double)accS[m * W + q] + (double)accC[m * W + q];
+                const double v = acc[m * W + q];
                 float *dst = A.map32[m] + pix0 + q;
                 *dst = A.accumulate ? (float)((double)*dst + v) : (float)v;
             }
@@ -1304,22 +1336,22 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
         }
         /* flush the deferred-pixel buffer when half full */
         if (sl->nfb >= FB_LDS / 2) {
-            uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
-            if (threadIdx.x == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
+            const uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
+            if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
             __syncthreads();
-            for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x)
+            for (uint32_t k = tid; k < nf; k += blockDim.x)
                 if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
             __syncthreads();
-            if (threadIdx.x == 0) sl->nfb = 0;
+            if (tid == 0) sl->nfb = 0;
         }
         __syncthreads();
     }
     {
-        uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
+        const uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
         if (nf) {
-            if (threadIdx.x == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
+            if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
             __syncthreads();
-            for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x)
+            for (uint32_t k = tid; k < nf; k += blockDim.x)
                 if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
         }
     }
@@ -1343,7 +1375,7 @@ __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
          i += (unsigned long long)gridDim.x * blockDim.x) {
         const uint4 e = fbr[i];
         const int32_t ring = (int32_t)e.x, ip = (int32_t)e.y;
-        const HaloRec &h = A.halos[e.z];
+        const HaloRec &h = A.halos[A.order[e.z]];
         int32_t npr, kshift; int64_t first;
         ring_info(A.nside, ring, npr, kshift, first);
         double zr, sth, off, den;
@@ -1377,12 +1409,13 @@ int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership)
+size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int nrt)
 {
     size_t W = (size_t)1 << log2w;
     size_t b = (size_t)n_maps * W * 8;
     if (membership) b += W * 12;
-    b += mode == 2 ? sizeof(FastSeg) : sizeof(SegLDS);
+    if (mode == 2) b = ((b + 15) & ~(size_t)15) + sizeof(FastSeg) + sizeof(float) * (size_t)(nrt + 2);
+    else b += sizeof(SegLDS);
     return b;
 }
 
@@ -1394,7 +1427,7 @@ static void launch_tile(const TileArgs &a, int grid, int wg, size_t lds, hipStre
 
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s)
 {
-    size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr);
+    size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr, a.g.nrt);
     if (lds > 160 * 1024) return -1;
 #define P2M_TL(MD, NMv) case NMv: launch_tile<MD, NMv>(a, grid, wg, lds, s); break;
 #define P2M_TF(NMv) case NMv: hipLaunchKernelGGL((k_tile_fast<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
