@@ -92,6 +92,7 @@ struct p2m_ctx {
     std::vector<int32_t> tile_base;
     int32_t *d_tile_base = nullptr;
     int32_t *d_tile_ring = nullptr;
+    double *d_ring_z = nullptr;     /* ring2z of every ring */
     uint32_t *d_group = nullptr;    /* halo of every 64th (halo, ring) item */
     int64_t group_cap = 0;
     uint32_t *d_fs_cnt = nullptr;   /* flat-sky box pixels per listed halo, and their scan */
@@ -182,6 +183,11 @@ static int build_tiles(p2m_ctx *c, int log2w)
     if (rc) return P2M_E_NOMEM;
     HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_tile_ring, tile_ring.data(), sizeof(int32_t) * acc, hipMemcpyHostToDevice));
+    if (!c->d_ring_z) {
+        if (dmalloc((void **)&c->d_ring_z, sizeof(double) * c->n_rings)) return P2M_E_NOMEM;
+        p2m_launch_ring_z(ns, c->d_ring_z, c->stream);
+        HIPCHK(hipGetLastError());
+    }
     c->scan_tmp_bytes = 0;
     c->log2w = log2w;
     return 0;
@@ -269,7 +275,7 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
-    dfree(c->d_tile_ring); dfree(c->d_group); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
+    dfree(c->d_tile_ring); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
     dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
     for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
@@ -594,7 +600,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         b.halos = c->d_bin; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
         b.nr = c->d_nr; b.ring_off = c->d_ring_off;
         b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt; b.tile_fill = c->d_tile_fill;
-        b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc;
+        b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc; b.ring_z = c->d_ring_z;
         /* (halo, ring) item space */
         p2m_launch_ring_counts(b, c->stream);
         { int rc = scan_counts(c, c->d_nr, c->d_ring_off, n_halo + 1); if (rc) return rc; }

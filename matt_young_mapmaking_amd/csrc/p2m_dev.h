@@ -284,9 +284,18 @@ P2M_DEV int64_t mod_ring(int64_t x, int64_t n)
  * of pixels np (0 if none) and the first in-ring index lo (pixels lo..lo+np-1
  * modulo npr). */
 template <typename H>
+P2M_DEV int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo);
+
+template <typename H>
 P2M_DEV int32_t disc_ring_run(const H &h, int32_t nside, int32_t ring, int32_t npr, int32_t kshift, int32_t &lo)
 {
-    double z = ring2z(nside, ring);
+    return disc_ring_run_z(h, ring2z(nside, ring), npr, kshift, lo);
+}
+
+/* the same, with the ring's z (ring2z) given */
+template <typename H>
+P2M_DEV int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo)
+{
     double dphi = disc_dphi(h, z);
     if (dphi >= HPX_PI) { lo = 0; return npr; }
     if (!(dphi >= 0.0)) return 0;

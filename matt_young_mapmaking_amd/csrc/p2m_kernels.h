@@ -69,6 +69,7 @@ struct BinArgs {
     const unsigned long long *ring_off;  /* exclusive scan of nr */
     unsigned long long n_items;   /* total (halo, ring) items */
     uint32_t *group_halo;         /* halo of item 64 g (n_items / 64 + 1 entries) */
+    const double *ring_z;         /* ring2z of every ring */
     uint2 *runs;                  /* per item: (first in-ring pixel, pixel count) */
     const int32_t *tile_base;     /* first tile of ring r+1 */
     uint32_t *tile_cnt;
@@ -132,6 +133,7 @@ void p2m_launch_paint_fs(const PaintArgs &a, int plain, hipStream_t s);
 void p2m_launch_fs_counts(const PaintArgs &a, hipStream_t s);
 void p2m_launch_fs_paint(const PaintArgs &a, hipStream_t s);
 void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s);
+void p2m_launch_ring_z(int32_t nside, double *z, hipStream_t s);
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
 uint32_t p2m_key_max(int32_t nside, int32_t nphi);

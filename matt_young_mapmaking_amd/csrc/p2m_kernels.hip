@@ -526,9 +526,10 @@ __global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
  * whose tile finds no hash slot within BIN_PROBE probes, or that finds the
  * piece buffer full, takes the direct path (one global atomic). */
 #define BIN_CH 1024
-#define BIN_HASH 2048
+#define BIN_HASH 1024
+#define BIN_HASH_BITS 10
 #define BIN_PROBE 32
-#define BIN_PIECES 2048
+#define BIN_PIECES 1280
 #define BIN_EMPTY 0xffffffffu
 
 struct BinHash {
@@ -544,7 +545,7 @@ struct BinLDS : BinHash {
 
 __device__ static inline int bin_slot(BinHash *L, uint32_t tile)
 {
-    uint32_t h = (tile * 2654435761u) >> 21;            /* 11 bits: BIN_HASH = 2048 */
+    uint32_t h = (tile * 2654435761u) >> (32 - BIN_HASH_BITS);
     for (int k = 0; k < BIN_PROBE; ++k) {
         const uint32_t prev = atomicCAS(&L->key[h], BIN_EMPTY, tile);
         if (prev == BIN_EMPTY || prev == tile) return (int)h;
@@ -584,6 +585,19 @@ __device__ static inline void wave_range(const unsigned long long *off, const ui
     }
 }
 
+/* z of every ring centre (ring2z) */
+__global__ void k_ring_z(int32_t nside, double *z)
+{
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (r <= 4 * nside - 1) z[r - 1] = ring2z(nside, r);
+}
+
+void p2m_launch_ring_z(int32_t nside, double *z, hipStream_t s)
+{
+    const int n = 4 * nside - 1;
+    hipLaunchKernelGGL(k_ring_z, dim3((n + 255) / 256), dim3(256), 0, s, nside, z);
+}
+
 /* group_halo[g] = the halo holding item 64 g */
 __global__ void __launch_bounds__(256) k_group_heads(BinArgs B)
 {
@@ -614,7 +628,7 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
             int32_t npr, kshift; int64_t first;
             if (ring <= h.irmax) {
                 ring_info(B.nside, ring, npr, kshift, first);
-                np = disc_ring_run(h, B.nside, ring, npr, kshift, lo);
+                np = disc_ring_run_z(h, B.ring_z[ring - 1], npr, kshift, lo);
             }
             B.runs[item] = make_uint2((uint32_t)lo, (uint32_t)np);
             if (np > 0) {
@@ -1263,6 +1277,8 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 const int s = segb[p];
                 const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
                 const int ip = ip0 + q;
+                /* azimuth offset in FP64 (the difference of two angles near
+                 * each other), folded by the period pi of sin^2 */
                 double x = 0.5 * __builtin_fma((double)ip, kphi, sl->dphi0[s]);
                 if (x > HPX_HALFPI) x -= HPX_PI;
                 const float s2 = sin2f((float)x);
