@@ -64,7 +64,7 @@ struct p2m_ctx {
     float hdr_f[6] = {0, 0, 0, 0, 0, 0};
     TabGeom g;
     float *d_flat[P2M_MAXMAPS] = {nullptr};
-    float *d_prof[P2M_MAXMAPS] = {nullptr};
+    float *d_prof[P2M_MAXMAPS] = {nullptr};    /* the profile's slice, zero-padded (FP32 painter) */
     int64_t n_flat = 0, n_prof = 0;
     double *d_inv_node2 = nullptr;
     float *d_inv_node2f = nullptr;
@@ -404,15 +404,20 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     /* padding covers every index interpolate_table can form past the end (Q6) */
     int64_t pad = 2LL * g.nrt * (g.nmht + 2) + 4LL * g.nrt + 64;
     c->n_flat = nt + pad;
-    c->n_prof = nt / 2 + pad / 2;
     if (!c->d_flat[slot] && dmalloc((void **)&c->d_flat[slot], sizeof(float) * c->n_flat)) return P2M_E_NOMEM;
-    if (!c->d_prof[slot] && dmalloc((void **)&c->d_prof[slot], sizeof(float) * c->n_prof)) return P2M_E_NOMEM;
-    std::vector<float> flat(c->n_flat, 0.f), prof(c->n_prof, 0.f);
+    std::vector<float> flat(c->n_flat, 0.f);
     memcpy(flat.data(), table, sizeof(float) * nt);
-    int p = c->cfg.profile;
-    for (int64_t i = 0; i < nt / 2; ++i) prof[i] = table[(p - 1) + 2 * i];
     HIPCHK(hipMemcpy(c->d_flat[slot], flat.data(), sizeof(float) * c->n_flat, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_prof[slot], prof.data(), sizeof(float) * c->n_prof, hipMemcpyHostToDevice));
+    if (c->cfg.precision == P2M_PREC_FP32_COMP) {
+        /* the profile's slice (column-major ir, imh, ichi), with the same
+         * zero padding past the end for the upper-edge aliasing (Q6) */
+        c->n_prof = nt / 2 + pad / 2;
+        std::vector<float> prof((size_t)c->n_prof, 0.f);
+        const int p = c->cfg.profile;
+        for (int64_t i = 0; i < nt / 2; ++i) prof[(size_t)i] = table[(p - 1) + 2 * i];
+        if (!c->d_prof[slot] && dmalloc((void **)&c->d_prof[slot], sizeof(float) * c->n_prof)) return P2M_E_NOMEM;
+        HIPCHK(hipMemcpy(c->d_prof[slot], prof.data(), sizeof(float) * c->n_prof, hipMemcpyHostToDevice));
+    }
     /* FP32 path constants: node radii, ln r -> cell coordinate */
     std::vector<double> inv(g.nrt + 2);
     for (int k = 0; k < g.nrt + 2; ++k) inv[k] = p2m_exp(-2.0 * (g.lr0 + (double)k * g.drt));

@@ -79,7 +79,7 @@ struct __attribute__((aligned(64))) TileHalo {
     float chi2;                  /* (float)(chi_r * chi_r) */
     float vh;                    /* velocity factor; 0 when chi > chimaxt */
     float fc, fm;                /* table fractions, rounded to float */
-    int32_t colbase;             /* nrt * ((imh - 1) + nmht * (ichi - 1)) */
+    int32_t colbase;             /* nrt * ((imh - 1) + nmht * (ichi - 1)); 0 when chi > chimaxt */
     uint32_t pad;
     unsigned long long hash;     /* membership hash of the catalogue index */
 };

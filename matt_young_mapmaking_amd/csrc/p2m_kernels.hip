@@ -742,7 +742,9 @@ __global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const u
             t.chi2 = (float)(h.chi_r * h.chi_r);
             t.vh = h.chi_out ? 0.0f : (float)h.vh;
             t.fc = (float)h.fc; t.fm = (float)h.fm;
-            t.colbase = nrt * ((h.imh - 1) + nmht * (h.ichi - 1));
+            /* a halo beyond the table's chi range paints zeros (vh = 0); its
+             * cell indices may lie outside the table, so point it at column 0 */
+            t.colbase = h.chi_out ? 0 : nrt * ((h.imh - 1) + nmht * (h.ichi - 1));
             t.pad = 0;
             t.hash = halo_hash(h.cat);
             hot[i] = t;
@@ -814,247 +816,191 @@ int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *
 /* tiled painter                                                          */
 /* ===================================================================== */
 
-#define SEG_BATCH 256
+/* ------------------------------------------------------------------ */
+/* tile painters: common parts                                          */
+/* ------------------------------------------------------------------ */
 
-/* per-segment constants staged in LDS */
-struct SegLDS {
-    uint32_t halo[SEG_BATCH];
-    uint32_t lo[SEG_BATCH];
-    uint32_t pre[SEG_BATCH + 1];
+#define FSEG 256                   /* segments staged per batch */
+#define FB_LDS 256                 /* deferred pixels buffered in LDS */
+#define PMAX 4096                  /* pixels per batch (>= the widest tile) */
+
+/* the part of a batch both painters share: segment starts and the pixel
+ * -> segment map */
+struct BatchLDS {
+    uint32_t pre[FSEG + 1];        /* exclusive scan of the staged lengths */
+    uint32_t seg_of[PMAX / 4];     /* batch pixel -> segment (bytes) */
+    uint32_t wsum[16];
+    uint32_t nb;
 };
 
-/* sin^2(x) for |x| <= pi/2 in FP64 (Taylor polynomials of p2m_math.h) */
-__device__ static inline double sin2_half(double x)
+/* Geometry of tile t: ring, first in-ring pixel ip0, width, first map pixel
+ * of the ring, and the pixel-centre z, sin(theta) and phi parameters. */
+struct TileGeo {
+    int32_t ring, ip0, width;
+    int64_t first;
+    double zr, sth, off, den;
+};
+
+__device__ static inline TileGeo tile_geo(const TileArgs &A, int64_t t, int W)
 {
-    double ax = p2m_fabs(x);
-    if (ax <= 0.7853981633974483) {
-        double s = p2m_sin_poly(x);
-        return s * s;
+    TileGeo g;
+    g.ring = A.tile_ring[t];
+    const int32_t col = (int32_t)(t - A.tile_base[g.ring - 1]);
+    int32_t npr, kshift;
+    ring_info(A.nside, g.ring, npr, kshift, g.first);
+    g.ip0 = col * W;
+    g.width = npr - g.ip0 < W ? npr - g.ip0 : W;
+    ring_zsth(A.nside, g.ring, g.zr, g.sth);
+    ring_phi_param(A.nside, g.ring, g.off, g.den);
+    return g;
+}
+
+/* XCD-aware tile order: workgroup b runs on XCD b % 8; the workgroups of
+ * one XCD sweep one contiguous eighth of the tiles (ring-major), so tiles in
+ * flight on an XCD are neighbours that share halos in its L2. */
+__device__ static inline void tile_range(const TileArgs &A, int64_t &tbeg, int64_t &tend, int64_t &tstep)
+{
+    const bool xcd = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
+    tstep = xcd ? gridDim.x / 8 : gridDim.x;
+    tbeg = xcd ? (int64_t)(blockIdx.x % 8) * A.n_tiles / 8 + blockIdx.x / 8 : blockIdx.x;
+    tend = xcd ? (int64_t)(blockIdx.x % 8 + 1) * A.n_tiles / 8 : A.n_tiles;
+}
+
+/* Given this thread's staged length (0 if none; nstage segments staged),
+ * forms pre[], takes the leading segments whose pixels fit in PMAX (a
+ * segment is at most W <= PMAX pixels, so at least one) and builds the
+ * pixel -> segment map.  Returns the number of segments in the batch; P
+ * receives its pixel count.  Every thread calls; ends with a barrier. */
+__device__ static inline int batch_prepare(BatchLDS *bl, uint32_t len, int nstage, uint32_t &P)
+{
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    uint32_t v = len;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        if (lane >= o) v += u;
     }
-    double y = (P2M_PIO2_HI - ax) + P2M_PIO2_LO;
-    double cc = p2m_cos_poly(y);
-    return cc * cc;
+    if (lane == 63) bl->wsum[wv] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int k = 0; k < wv; ++k) add += bl->wsum[k];
+    v += add;
+    if (tid < nstage) bl->pre[tid + 1] = v;
+    if (tid == 0) bl->pre[0] = 0;
+    __syncthreads();
+    if (tid < nstage) {
+        const uint32_t e = bl->pre[tid + 1];
+        if (e <= PMAX && (tid + 1 == nstage || bl->pre[tid + 2] > PMAX)) bl->nb = (uint32_t)(tid + 1);
+    }
+    __syncthreads();
+    const int nb = (int)bl->nb;
+    P = bl->pre[nb];
+    /* each thread takes a run of whole words, finds the segment of its first
+     * pixel by binary search and walks forward */
+    const uint32_t nw = (P + 3) >> 2;
+    const uint32_t wpt = (nw + blockDim.x - 1) / blockDim.x;
+    const uint32_t w0 = tid * wpt;
+    const uint32_t w1 = w0 + wpt < nw ? w0 + wpt : nw;
+    if (w0 < w1) {
+        const uint32_t p0 = w0 * 4;
+        int lo_s = 0, hi_s = nb - 1;
+        while (lo_s < hi_s) {
+            const int mid = (lo_s + hi_s + 1) >> 1;
+            if (bl->pre[mid] <= p0) lo_s = mid; else hi_s = mid - 1;
+        }
+        uint32_t sg = (uint32_t)lo_s, nxt = bl->pre[sg + 1];
+        for (uint32_t w = w0; w < w1; ++w) {
+            uint32_t o = 0;
+            for (int bb = 0; bb < 4; ++bb) {
+                const uint32_t p = w * 4 + bb;
+                while (p >= nxt && (int)sg + 1 < nb) { ++sg; nxt = bl->pre[sg + 1]; }
+                o |= sg << (8 * bb);
+            }
+            bl->seg_of[w] = o;
+        }
+    }
+    __syncthreads();
+    return nb;
 }
 
-/* ln(1+q) for 0 <= q < 0.08 in FP32: 2 atanh(q/(2+q)) */
-__device__ static inline float log1p_small(float q)
-{
-    float s = q / (2.0f + q);
-    float s2 = s * s;
-    float p = __builtin_fmaf(s2, 1.0f / 7.0f, 1.0f / 5.0f);
-    p = __builtin_fmaf(s2, p, 1.0f / 3.0f);
-    return 2.0f * __builtin_fmaf(s * s2, p, s);
-}
+/* ------------------------------------------------------------------ */
+/* FP64 tiled painter                                                   */
+/*                                                                      */
+/* The exact arithmetic of the reference per pixel: pix2vec_ring from    */
+/* the ring's z and sin(theta) and the pixel's phi, angdist against the  */
+/* disc radius, interpolate_table in FP64, accumulated in an FP64 LDS    */
+/* tile (ds_add_f64) and reduced to the FP64 map.                        */
+/* ------------------------------------------------------------------ */
 
-template <int MODE, int NM>
-__global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
+struct ExactSeg {
+    BatchLDS b;
+    uint32_t halo[FSEG];
+    uint32_t lo[FSEG];
+};
+
+template <int NM>
+__global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
 {
     extern __shared__ unsigned char smem_raw[];
     const int W = 1 << A.log2w;
-    /* LDS layout: acc (NM*W doubles, or NM*W*2 floats), count (W u32), hash (W u64), segs */
-    double *acc64 = (double *)smem_raw;
-    float *accS = (float *)smem_raw;
-    float *accC = accS + NM * W;
-    size_t acc_bytes = (size_t)NM * W * 8;
-    uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes);
-    unsigned long long *lhash = (unsigned long long *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 4 : 0));
-    size_t mem_bytes = A.count ? (size_t)W * 12 : 0;
-    SegLDS *sl = (SegLDS *)(smem_raw + acc_bytes + mem_bytes);
-
-    unsigned long long n_upd = 0, n_fb = 0;
-    /* XCD-aware order: workgroup b runs on XCD b % 8; the workgroups of one
-     * XCD sweep one contiguous eighth of the tiles (ring-major), so tiles
-     * in flight on an XCD are neighbours that share halos in its L2 */
-    const bool xcd = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
-    const int64_t tstep = xcd ? gridDim.x / 8 : gridDim.x;
-    const int64_t tbeg = xcd ? (int64_t)(blockIdx.x % 8) * A.n_tiles / 8 + blockIdx.x / 8 : blockIdx.x;
-    const int64_t tend = xcd ? (int64_t)(blockIdx.x % 8 + 1) * A.n_tiles / 8 : A.n_tiles;
+    /* LDS: acc (NM*W doubles) | hash (W u64) count (W u32) if membership | ExactSeg */
+    double *acc = (double *)smem_raw;
+    const size_t acc_bytes = (size_t)NM * W * 8;
+    unsigned long long *lhash = (unsigned long long *)(smem_raw + acc_bytes);
+    uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 8 : 0));
+    const size_t mem_bytes = A.count ? (size_t)W * 12 : 0;
+    ExactSeg *sl = (ExactSeg *)(smem_raw + ((acc_bytes + mem_bytes + 15) & ~(size_t)15));
+    __shared__ unsigned long long lds16[16];
+    const int tid = threadIdx.x;
+    unsigned long long n_upd = 0;
+    int64_t tbeg, tend, tstep;
+    tile_range(A, tbeg, tend, tstep);
     for (int64_t t = tbeg; t < tend; t += tstep) {
-        /* ring of this tile: largest r with tile_base[r] <= t */
-        int lo_r = 0, hi_r = A.n_rings - 1;
-        while (lo_r < hi_r) {
-            int mid = (lo_r + hi_r + 1) >> 1;
-            if (A.tile_base[mid] <= t) lo_r = mid; else hi_r = mid - 1;
-        }
-        const int32_t ring = lo_r + 1;
-        const int32_t col = (int32_t)(t - A.tile_base[lo_r]);
-        int32_t npr, kshift; int64_t first;
-        ring_info(A.nside, ring, npr, kshift, first);
-        const int32_t ip0 = col * W;
-        const int32_t width = npr - ip0 < W ? npr - ip0 : W;
-        double zr, sth;
-        ring_zsth(A.nside, ring, zr, sth);
-        double off, den;
-        ring_phi_param(A.nside, ring, off, den);
-
-        for (int q = threadIdx.x; q < width; q += blockDim.x) {
-            for (int m = 0; m < NM; ++m) {
-                if (MODE == 1) acc64[m * W + q] = 0.0;
-                else { accS[m * W + q] = 0.0f; accC[m * W + q] = 0.0f; }
-            }
+        const TileGeo G = tile_geo(A, t, W);
+        for (int q = tid; q < G.width; q += blockDim.x) {
+            for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
-
         const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
         for (uint64_t b0 = sbeg; b0 < send;) {
             __syncthreads();
-            int nb = (int)(send - b0 < SEG_BATCH ? send - b0 : SEG_BATCH);
-            /* stage the batch and scan its lengths (simple two-level scan) */
-            const int cap = (int)blockDim.x < SEG_BATCH ? (int)blockDim.x : SEG_BATCH;
-            if (nb > cap) nb = cap;
+            int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
+            if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
-            if ((int)threadIdx.x < nb) {
-                uint2 sg = A.segs[b0 + threadIdx.x];
-                sl->halo[threadIdx.x] = sg.x;
-                sl->lo[threadIdx.x] = sg.y >> 16;
+            if (tid < nstage) {
+                const uint2 sg = A.segs[b0 + tid];
+                sl->halo[tid] = sg.x;
+                sl->lo[tid] = sg.y >> 16;
                 len = sg.y & 0xffffu;
             }
-            /* inclusive wave scan */
-            uint32_t v = len;
-            int lane = __lane_id();
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t u = __shfl_up(v, o);
-                if (lane >= o) v += u;
-            }
-            __shared__ uint32_t wsum[16];
-            int wv = threadIdx.x >> 6;
-            if (lane == 63) wsum[wv] = v;
-            __syncthreads();
-            uint32_t add = 0;
-            for (int k = 0; k < wv; ++k) add += wsum[k];
-            if ((int)threadIdx.x < nb) sl->pre[threadIdx.x + 1] = v + add;
-            if (threadIdx.x == 0) sl->pre[0] = 0;
-            __syncthreads();
-            const uint32_t P = sl->pre[nb];
+            uint32_t P;
+            const int nb = batch_prepare(&sl->b, len, nstage, P);
             b0 += (uint64_t)nb;
-            for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-                /* segment of pixel p: largest s with pre[s] <= p */
-                int s_lo = 0, s_hi = nb - 1;
-                while (s_lo < s_hi) {
-                    int mid = (s_lo + s_hi + 1) >> 1;
-                    if (sl->pre[mid] <= p) s_lo = mid; else s_hi = mid - 1;
-                }
-                const int s = s_lo;
-                const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
+            const unsigned char *segb = (const unsigned char *)sl->b.seg_of;
+            for (uint32_t p = tid; p < P; p += blockDim.x) {
+                const int s = segb[p];
+                const int q = (int)(sl->lo[s] + (p - sl->b.pre[s]));
                 const HaloRec &h = A.halos[sl->halo[s]];
-                const double ipd = (double)(ip0 + q);
-                const double phi = ((ipd + off) * HPX_HALFPI) / den;
-                bool member = false;
-                double vexact[NM];
-                float v32[NM];
-                bool use_exact = (MODE == 1);
-                if (MODE == 2) {
-                    double dphi = phi - h.phi0;
-                    if (dphi > HPX_PI) dphi -= HPX_TWOPI;
-                    double s2 = sin2_half(0.5 * dphi);
-                    double dz = zr - h.zz0, ds = sth - h.s0;
-                    double c2 = (dz * dz + ds * ds) + ((4.0 * sth) * h.s0) * s2;
-                    if (c2 >= h.thr * (1.0 + 1e-9)) {
-                        member = false;
-                    } else if (c2 > h.thr * (1.0 - 1e-9)) {
-                        use_exact = true;
-                    } else {
-                        member = true;
-                        double h2 = 0.25 * c2;
-                        double dd = 1.0 - 2.0 * h2;
-                        if (h.chi_out) {
-                            for (int m = 0; m < NM; ++m) v32[m] = 0.0f;
-                        } else if (!(dd > 0.0)) {
-                            use_exact = true;
-                        } else {
-                            double t2 = ((4.0 * h2) * (1.0 - h2)) / (dd * dd);
-                            double r2 = (h.chi_r * h.chi_r) * t2;
-                            float u = __builtin_amdgcn_logf((float)r2) * A.u_k1 - A.u_k0;
-                            float fl = __builtin_floorf(u);
-                            float fu = u - fl;
-                            if (u > A.u_max + A.band) {
-                                for (int m = 0; m < NM; ++m) v32[m] = 0.0f;   /* r > rmaxt */
-                            } else if ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
-                                       (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))) {
-                                use_exact = true;
-                            } else {
-                                int ir;
-                                float fr;
-                                if (u < 0.0f) {      /* r < rmint: r = rmint + 1e-5 (maptable.f90:190) */
-                                    ir = 1;
-                                    fr = A.fr_clamp;
-                                } else {
-                                    ir = (int)fl + 1;
-                                    double q2 = r2 * A.inv_node2[ir - 1] - 1.0;
-                                    fr = 0.5f * log1p_small((float)q2);
-                                }
-                                int64_t col0 = (int64_t)A.g.nrt * ((int64_t)(h.imh - 1) + (int64_t)A.g.nmht * (int64_t)(h.ichi - 1)) + (ir - 1);
-                                int64_t dmh = A.g.nrt, dch = (int64_t)A.g.nrt * A.g.nmht;
-                                float fc = (float)h.fc, fm = (float)h.fm;
-                                float w00 = (1.0f - fc) * (1.0f - fm), w10 = fc * (1.0f - fm);
-                                float w01 = (1.0f - fc) * fm, w11 = fc * fm;
-                                float vh = (float)h.vh;
-                                for (int m = 0; m < NM; ++m) {
-                                    const float *T = A.tab_prof[m];
-                                    float a0 = T[col0], a1 = T[col0 + 1];
-                                    float b0v = T[col0 + dmh], b1 = T[col0 + dmh + 1];
-                                    float c0v = T[col0 + dch], c1 = T[col0 + dch + 1];
-                                    float d0 = T[col0 + dmh + dch], d1 = T[col0 + dmh + dch + 1];
-                                    float ea = __builtin_fmaf(fr, a1 - a0, a0);
-                                    float eb = __builtin_fmaf(fr, b1 - b0v, b0v);
-                                    float ec = __builtin_fmaf(fr, c1 - c0v, c0v);
-                                    float ed = __builtin_fmaf(fr, d1 - d0, d0);
-                                    float val = w00 * ea + w10 * eb + w01 * ec + w11 * ed;
-                                    v32[m] = vh * val;
-                                }
-                            }
-                        }
-                    }
-                }
-                if (use_exact) {
-                    double s, cph;
-                    p2m_sincos(phi, &s, &cph);
-                    double angle = angdist(h.hx, h.hy, h.hz, sth * cph, sth * s, zr);
-                    member = angle < h.dtheta;
-                    if (member) {
-                        for (int m = 0; m < NM; ++m)
-                            vexact[m] = h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle);
-                        if (MODE == 2) {
-                            n_fb++;
-                            for (int m = 0; m < NM; ++m) v32[m] = (float)vexact[m];
-                        }
-                    }
-                }
-                if (member) {
-                    n_upd++;
-                    for (int m = 0; m < NM; ++m) {
-                        if (MODE == 1) {
-                            atomicAdd(&acc64[m * W + q], vexact[m]);
-                        } else {
-                            float vv = v32[m];
-                            float old = atomicAdd(&accS[m * W + q], vv);
-                            float sum = old + vv;
-                            float bb = sum - old;
-                            float err = (old - (sum - bb)) + (vv - bb);
-                            atomicAdd(&accC[m * W + q], err);
-                        }
-                    }
-                    if (A.count) {
-                        atomicAdd(&lcount[q], 1u);
-                        atomicAdd(&lhash[q], (unsigned long long)halo_hash(h.cat));
-                    }
+                const double phi = (((double)(G.ip0 + q) + G.off) * HPX_HALFPI) / G.den;
+                double sn, cs;
+                p2m_sincos(phi, &sn, &cs);
+                const double angle = angdist(h.hx, h.hy, h.hz, G.sth * cs, G.sth * sn, G.zr);
+                if (!(angle < h.dtheta)) continue;
+                n_upd++;
+                for (int m = 0; m < NM; ++m)
+                    atomicAdd(&acc[m * W + q], h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle));
+                if (A.count) {
+                    atomicAdd(&lcount[q], 1u);
+                    atomicAdd(&lhash[q], (unsigned long long)halo_hash(h.cat));
                 }
             }
         }
         __syncthreads();
-        /* tile-to-map reduce */
-        const int64_t pix0 = first + ip0;
-        for (int q = threadIdx.x; q < width; q += blockDim.x) {
+        const int64_t pix0 = G.first + G.ip0;
+        for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) {
-                if (MODE == 1) {
-                    double v = acc64[m * W + q];
-                    double *dst = A.map64[m] + pix0 + q;
-                    *dst = A.accumulate ? *dst + v : v;
-                } else {
-                    double v = (double)accS[m * W + q] + (double)accC[m * W + q];
-                    float *dst = A.map32[m] + pix0 + q;
-                    *dst = A.accumulate ? (float)((double)*dst + v) : (float)v;
-                }
+                const double v = acc[m * W + q];
+                double *dst = A.map64[m] + pix0 + q;
+                *dst = A.accumulate ? *dst + v : v;
             }
             if (A.count) {
                 uint32_t *dc = A.count + pix0 + q;
@@ -1065,9 +1011,7 @@ __global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
         }
         __syncthreads();
     }
-    __shared__ unsigned long long lds16[16];
     block_count(&A.counters[1], n_upd, lds16);
-    block_count(&A.counters[5], n_fb, lds16);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1078,29 +1022,25 @@ __global__ void __launch_bounds__(512) k_tile_paint(TileArgs A)
 /* constants A = (z-z0)^2 + (sth-s0)^2, B = 4 sth s0 staged in LDS;        */
 /* tan^2(theta) = 4h(1-h)/(1-2h)^2 with h = chord^2/4; the cell coordinate */
 /* u = (ln r - ln rmin)/drt from log2(r^2); fr = ln(r/r_node) from         */
-/* log1p(r^2/r_node^2 - 1).  Pixels whose chord^2 lies within band_c of    */
-/* the disc edge, or whose u lies within `band` of a cell boundary, the    */
-/* r cut-off or the r clamp, are deferred to k_fallback, which decides     */
-/* them with the exact FP64 arithmetic of the reference.                   */
+/* log1p(r^2/r_node^2 - 1); a lerp of the 4 bracketing radial columns.    */
+/* Values are summed in an FP64 LDS tile.  Pixels whose                    */
+/* chord^2 lies within band_c of the disc edge, or whose u lies within     */
+/* `band` of a cell boundary, the r cut-off or the r clamp, are deferred   */
+/* to k_fallback, which decides them with the exact FP64 arithmetic of the */
+/* reference.                                                              */
 /* --------------------------------------------------------------------- */
 
-#define FSEG 256
-#define FB_LDS 256
-#define PMAX 4096                  /* pixels per staged batch (>= the widest tile) */
-
 struct FastSeg {
+    BatchLDS b;
     double dphi0[FSEG];            /* phi offset of the ring's pixel 0 from the halo centre */
     unsigned long long hh[FSEG];
     uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
     uint32_t lo[FSEG];
-    uint32_t pre[FSEG + 1];
     float A[FSEG], B[FSEG], thr_hi[FSEG], thr_lo[FSEG], chi2[FSEG], vh[FSEG];
     float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];
     int32_t colbase[FSEG];
-    uint32_t seg_of[PMAX / 4];     /* batch pixel -> segment (bytes) */
-    uint32_t wmax[16];
-    uint32_t nfb, nb;
+    uint32_t nfb;
 };
 
 /* sin^2(x), |x| <= pi/2, FP32 Taylor polynomials (to x^11 / x^10) */
@@ -1137,9 +1077,6 @@ __device__ static inline float log1p_small_rcp(float q)
     return 2.0f * __builtin_fmaf(s * s2, p, s);
 }
 
-/* byte k of a packed word */
-__device__ static inline uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xffu; }
-
 template <int NM>
 __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
 {
@@ -1155,11 +1092,10 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     float *invn = (float *)(sl + 1);
     __shared__ unsigned long long lds16[16];
     __shared__ unsigned long long fb_base;
-    __shared__ uint32_t wsum[16];
 
     const int nrt = A.g.nrt;
     const int64_t dmh = nrt, dch = (int64_t)nrt * A.g.nmht;
-    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6, nwv = blockDim.x >> 6;
+    const int tid = threadIdx.x;
     unsigned long long n_upd = 0, n_fb = 0;
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
     unsigned long long *const fbc = A.fb_count + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE;
@@ -1168,30 +1104,15 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     if (tid == 0) sl->nfb = 0;
     __syncthreads();
 
-    /* XCD-aware order: workgroup b runs on XCD b % 8; the workgroups of one
-     * XCD sweep one contiguous eighth of the tiles (ring-major), so tiles
-     * in flight on an XCD are neighbours that share halos in its L2 */
-    const bool xcd = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
-    const int64_t tstep = xcd ? gridDim.x / 8 : gridDim.x;
-    const int64_t tbeg = xcd ? (int64_t)(blockIdx.x % 8) * A.n_tiles / 8 + blockIdx.x / 8 : blockIdx.x;
-    const int64_t tend = xcd ? (int64_t)(blockIdx.x % 8 + 1) * A.n_tiles / 8 : A.n_tiles;
+    int64_t tbeg, tend, tstep;
+    tile_range(A, tbeg, tend, tstep);
     for (int64_t t = tbeg; t < tend; t += tstep) {
-        const int32_t ring = A.tile_ring[t];
-        const int32_t col = (int32_t)(t - A.tile_base[ring - 1]);
-        int32_t npr, kshift; int64_t first;
-        ring_info(A.nside, ring, npr, kshift, first);
-        const int32_t ip0 = col * W;
-        const int32_t width = npr - ip0 < W ? npr - ip0 : W;
-        double zr, sth, off, den;
-        ring_zsth(A.nside, ring, zr, sth);
-        ring_phi_param(A.nside, ring, off, den);
-        const double kphi = HPX_HALFPI / den;
-
-        for (int q = tid; q < width; q += blockDim.x) {
+        const TileGeo G = tile_geo(A, t, W);
+        const double kphi = HPX_HALFPI / G.den;
+        for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
-
         const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
         for (uint64_t b0 = sbeg; b0 < send;) {
             __syncthreads();
@@ -1204,9 +1125,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 sl->halo[tid] = sg.x;
                 sl->lo[tid] = sg.y >> 16;
                 len = sg.y & 0xffffu;
-                const double dz = zr - h.zz0, ds = sth - h.s0;
+                const double dz = G.zr - h.zz0, ds = G.sth - h.s0;
                 sl->A[tid] = (float)(dz * dz + ds * ds);
-                sl->B[tid] = (float)((4.0 * sth) * h.s0);
+                sl->B[tid] = (float)((4.0 * G.sth) * h.s0);
                 sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
                 sl->thr_lo[tid] = (float)(h.thr * (1.0 - (double)A.band_c));
                 sl->chi2[tid] = h.chi2;
@@ -1217,74 +1138,24 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 sl->w01[tid] = (1.0f - fc) * fm;
                 sl->w11[tid] = fc * fm;
                 sl->colbase[tid] = h.colbase;
-                sl->dphi0[tid] = off * kphi - h.phi0;
+                sl->dphi0[tid] = G.off * kphi - h.phi0;
                 sl->hh[tid] = h.hash;
             }
-            /* inclusive scan of the staged lengths */
-            uint32_t v = len;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t u = __shfl_up(v, o);
-                if (lane >= o) v += u;
-            }
-            if (lane == 63) wsum[wv] = v;
-                    __syncthreads();
-            uint32_t add = 0;
-            for (int k = 0; k < wv; ++k) add += wsum[k];
-            v += add;
-            if (tid < nstage) sl->pre[tid + 1] = v;
-            if (tid == 0) sl->pre[0] = 0;
-            __syncthreads();
-            /* the batch: the leading segments whose pixels fit in PMAX (a
-             * segment is at most W <= PMAX pixels, so at least one) */
-            if (tid < nstage) {
-                const uint32_t e = sl->pre[tid + 1];
-                if (e <= PMAX && (tid + 1 == nstage || sl->pre[tid + 2] > PMAX)) sl->nb = (uint32_t)(tid + 1);
-            }
-            __syncthreads();
-            const int nb = (int)sl->nb;
-            const uint32_t P = sl->pre[nb];
-            /* segment of every batch pixel, as bytes: each thread takes a run
-             * of whole words, finds the segment of its first pixel by binary
-             * search and walks forward */
-            {
-                const uint32_t nw = (P + 3) >> 2;
-                const uint32_t wpt = (nw + blockDim.x - 1) / blockDim.x;
-                const uint32_t w0 = tid * wpt;
-                const uint32_t w1 = w0 + wpt < nw ? w0 + wpt : nw;
-                if (w0 < w1) {
-                    const uint32_t p0 = w0 * 4;
-                    int lo_s = 0, hi_s = nb - 1;
-                    while (lo_s < hi_s) {
-                        const int mid = (lo_s + hi_s + 1) >> 1;
-                        if (sl->pre[mid] <= p0) lo_s = mid; else hi_s = mid - 1;
-                    }
-                    uint32_t sg = (uint32_t)lo_s, nxt = sl->pre[sg + 1];
-                    for (uint32_t w = w0; w < w1; ++w) {
-                        uint32_t o = 0;
-                        for (int bb = 0; bb < 4; ++bb) {
-                            const uint32_t p = w * 4 + bb;
-                            while (p >= nxt && (int)sg + 1 < nb) { ++sg; nxt = sl->pre[sg + 1]; }
-                            o |= sg << (8 * bb);
-                        }
-                        sl->seg_of[w] = o;
-                    }
-                }
-            }
+            uint32_t P;
+            const int nb = batch_prepare(&sl->b, len, nstage, P);
             b0 += (uint64_t)nb;
-            __syncthreads();
-            const unsigned char *segb = (const unsigned char *)sl->seg_of;
+            const unsigned char *segb = (const unsigned char *)sl->b.seg_of;
             for (uint32_t p = tid; p < P; p += blockDim.x) {
                 const int s = segb[p];
-                const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
-                const int ip = ip0 + q;
+                const int q = (int)(sl->lo[s] + (p - sl->b.pre[s]));
+                const int ip = G.ip0 + q;
                 /* azimuth offset in FP64 (the difference of two angles near
                  * each other), folded by the period pi of sin^2 */
                 double x = 0.5 * __builtin_fma((double)ip, kphi, sl->dphi0[s]);
                 if (x > HPX_HALFPI) x -= HPX_PI;
                 const float s2 = sin2f((float)x);
                 const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
-                const float thr_hi = sl->thr_hi[s];
-                if (!(c2 < thr_hi)) continue;                      /* outside the disc */
+                if (!(c2 < sl->thr_hi[s])) continue;              /* outside the disc */
                 const float h2 = 0.25f * c2;
                 const float dd = 1.0f - 2.0f * h2;
                 const float rd = __builtin_amdgcn_rcpf(dd);
@@ -1297,14 +1168,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 const bool defer = c2 > sl->thr_lo[s] || !(dd > 1e-3f) ||
                                    (!zero && ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
                                               (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))));
-                const bool clampr = u < 0.0f;                     /* r < rmint: r = rmint + 1e-5 */
-                int ir = clampr ? 1 : (int)fl + 1;
-                ir = ir < 1 ? 1 : (ir > nrt - 1 ? nrt - 1 : ir);
-                const float fr = clampr ? A.fr_clamp
-                                        : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 if (defer) {
                     const uint32_t slot = atomicAdd(&sl->nfb, 1u);
-                    const uint4 e = make_uint4((uint32_t)ring, (uint32_t)ip, sl->halo[s], 0u);
+                    const uint4 e = make_uint4((uint32_t)G.ring, (uint32_t)ip, sl->halo[s], 0u);
                     if (slot < FB_LDS) sl->fb[slot] = e;
                     else {
                         const unsigned long long gs = atomicAdd(fbc, 1ULL);
@@ -1313,6 +1179,11 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                     n_fb++;
                     continue;
                 }
+                const bool clampr = u < 0.0f;                     /* r < rmint: r = rmint + 1e-5 */
+                int ir = clampr ? 1 : (int)fl + 1;
+                ir = ir < 1 ? 1 : (ir > nrt - 1 ? nrt - 1 : ir);
+                const float fr = clampr ? A.fr_clamp
+                                        : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 const int64_t c0 = (int64_t)sl->colbase[s] + (ir - 1);
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
                 const float vh = zero ? 0.0f : sl->vh[s];
@@ -1336,8 +1207,8 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
             }
         }
         __syncthreads();
-        const int64_t pix0 = first + ip0;
-        for (int q = tid; q < width; q += blockDim.x) {
+        const int64_t pix0 = G.first + G.ip0;
+        for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) {
                 const double v = acc[m * W + q];
                 float *dst = A.map32[m] + pix0 + q;
@@ -1430,30 +1301,24 @@ size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int n
     size_t W = (size_t)1 << log2w;
     size_t b = (size_t)n_maps * W * 8;
     if (membership) b += W * 12;
-    if (mode == 2) b = ((b + 15) & ~(size_t)15) + sizeof(FastSeg) + sizeof(float) * (size_t)(nrt + 2);
-    else b += sizeof(SegLDS);
+    b = (b + 15) & ~(size_t)15;
+    if (mode == 2) b += sizeof(FastSeg) + sizeof(float) * (size_t)(nrt + 2);
+    else b += sizeof(ExactSeg);
     return b;
-}
-
-template <int MODE, int NM>
-static void launch_tile(const TileArgs &a, int grid, int wg, size_t lds, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_tile_paint<MODE, NM>), dim3(grid), dim3(wg), lds, s, a);
 }
 
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s)
 {
     size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr, a.g.nrt);
-    if (lds > 160 * 1024) return -1;
-#define P2M_TL(MD, NMv) case NMv: launch_tile<MD, NMv>(a, grid, wg, lds, s); break;
+    if (lds > 160 * 1024 || wg > 512 || (mode == 2 && wg > 256)) return -1;
+#define P2M_TE(NMv) case NMv: hipLaunchKernelGGL((k_tile_exact<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
 #define P2M_TF(NMv) case NMv: hipLaunchKernelGGL((k_tile_fast<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
     if (mode == 1) {
-        switch (n_maps) { P2M_TL(1, 1) P2M_TL(1, 2) P2M_TL(1, 3) P2M_TL(1, 4) P2M_TL(1, 5) P2M_TL(1, 6) default: return -1; }
+        switch (n_maps) { P2M_TE(1) P2M_TE(2) P2M_TE(3) P2M_TE(4) P2M_TE(5) P2M_TE(6) default: return -1; }
     } else {
-        if (wg > 256) return -1;
         switch (n_maps) { P2M_TF(1) P2M_TF(2) P2M_TF(3) P2M_TF(4) P2M_TF(5) P2M_TF(6) default: return -1; }
     }
-#undef P2M_TL
+#undef P2M_TE
 #undef P2M_TF
     return 0;
 }
