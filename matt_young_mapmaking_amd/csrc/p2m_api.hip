@@ -92,6 +92,7 @@ struct p2m_ctx {
     std::vector<int32_t> tile_base;
     int32_t *d_tile_base = nullptr;
     int32_t *d_tile_ring = nullptr;
+    TileDesc *d_tile_desc = nullptr;
     double *d_ring_z = nullptr;     /* ring2z of every ring */
     uint32_t *d_group = nullptr;    /* halo of every 64th (halo, ring) item */
     int64_t group_cap = 0;
@@ -172,10 +173,12 @@ static int build_tiles(p2m_ctx *c, int log2w)
         for (int64_t t = c->tile_base[r - 1]; t < e; ++t) tile_ring[(size_t)t] = r;
     }
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_tile_ring);
+    dfree(c->d_tile_desc);
     c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_fill = nullptr; c->d_tile_off = nullptr;
-    c->d_tile_ring = nullptr;
+    c->d_tile_ring = nullptr; c->d_tile_desc = nullptr;
     int rc = 0;
     rc |= dmalloc((void **)&c->d_tile_ring, sizeof(int32_t) * acc);
+    rc |= dmalloc((void **)&c->d_tile_desc, sizeof(TileDesc) * acc);
     rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
     rc |= dmalloc((void **)&c->d_tile_cnt, sizeof(uint32_t) * (c->n_tiles + 1));
     rc |= dmalloc((void **)&c->d_tile_fill, sizeof(uint32_t) * (c->n_tiles + 1));
@@ -183,6 +186,8 @@ static int build_tiles(p2m_ctx *c, int log2w)
     if (rc) return P2M_E_NOMEM;
     HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_tile_ring, tile_ring.data(), sizeof(int32_t) * acc, hipMemcpyHostToDevice));
+    p2m_launch_tile_desc(ns, log2w, acc, c->d_tile_ring, c->d_tile_base, c->d_tile_desc, c->stream);
+    HIPCHK(hipGetLastError());
     if (!c->d_ring_z) {
         if (dmalloc((void **)&c->d_ring_z, sizeof(double) * c->n_rings)) return P2M_E_NOMEM;
         p2m_launch_ring_z(ns, c->d_ring_z, c->stream);
@@ -275,7 +280,7 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
-    dfree(c->d_tile_ring); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
+    dfree(c->d_tile_ring); dfree(c->d_tile_desc); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
     dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
     for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
@@ -659,7 +664,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         memset(&t, 0, sizeof(t));
         t.halos = c->d_halo_s; t.hot = c->d_hot; t.nside = c->cfg.nside; t.log2w = c->log2w; t.n_rings = c->n_rings;
         t.n_maps = c->cfg.n_maps; t.profile = c->cfg.profile; t.accumulate = c->hp_zero ? 0 : 1;
-        t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.tile_ring = c->d_tile_ring; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
+        t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.desc = c->d_tile_desc; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
         t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
         t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
         t.band_c = 4e-6f;

@@ -70,7 +70,7 @@ struct BinArgs {
     unsigned long long n_items;   /* total (halo, ring) items */
     uint32_t *group_halo;         /* halo of item 64 g (n_items / 64 + 1 entries) */
     const double *ring_z;         /* ring2z of every ring */
-    uint2 *runs;                  /* per item: (first in-ring pixel, pixel count) */
+    uint2 *runs;                  /* per item: (first in-ring pixel | ring << 16, pixel count) */
     const int32_t *tile_base;     /* first tile of ring r+1 */
     uint32_t *tile_cnt;
     uint32_t *tile_fill;
@@ -79,13 +79,23 @@ struct BinArgs {
     unsigned long long *counters; /* [0] listed, [3] ring gaps, [4] segments */
 };
 
+/* Per-tile geometry, computed once per tile shape: the pixel-centre z and
+ * sin(theta) of the tile's ring, its phi parameters (phi = (ip + off)
+ * pi/2 / den, kphi = pi/2 / den), the ring's first map pixel, the ring,
+ * the tile's first in-ring pixel and its width. */
+struct __attribute__((aligned(16))) TileDesc {
+    double zr, sth, off, den, kphi;
+    int64_t first;
+    int32_t ring, ip0, width, pad;
+};
+
 struct TileArgs {
     const HaloRec *halos;
     const TileHalo *hot;          /* the same halos, packed for the FP32 painter */
     int32_t nside, log2w, n_rings, n_maps, profile, accumulate;
     int64_t n_tiles;
     const int32_t *tile_base;
-    const int32_t *tile_ring;     /* ring of each tile */
+    const TileDesc *desc;         /* geometry of each tile */
     const unsigned long long *tile_off;
     const uint2 *segs;
     TabGeom g;
@@ -134,6 +144,8 @@ void p2m_launch_fs_counts(const PaintArgs &a, hipStream_t s);
 void p2m_launch_fs_paint(const PaintArgs &a, hipStream_t s);
 void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s);
 void p2m_launch_ring_z(int32_t nside, double *z, hipStream_t s);
+void p2m_launch_tile_desc(int32_t nside, int32_t log2w, int64_t n_tiles, const int32_t *tile_ring,
+                          const int32_t *tile_base, TileDesc *desc, hipStream_t s);
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s);
 void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
 uint32_t p2m_key_max(int32_t nside, int32_t nphi);

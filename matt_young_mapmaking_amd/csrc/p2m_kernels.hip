@@ -630,7 +630,8 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
                 ring_info(B.nside, ring, npr, kshift, first);
                 np = disc_ring_run_z(h, B.ring_z[ring - 1], npr, kshift, lo);
             }
-            B.runs[item] = make_uint2((uint32_t)lo, (uint32_t)np);
+            /* (first pixel | ring << 16, count): both below 2^16 for nside <= 8192 */
+            B.runs[item] = make_uint2((uint32_t)lo | ((uint32_t)ring << 16), (uint32_t)np);
             if (np > 0) {
                 n_list += (unsigned long long)np;
                 run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t, int32_t) {
@@ -678,11 +679,10 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
                 }
                 return;
             }
-            const BinHalo &h = B.halos[hi];
-            const int32_t ring = h.irmin + (int32_t)k;
+            const int32_t ring = (int32_t)(run.x >> 16);
             int32_t npr, kshift; int64_t first;
             ring_info(B.nside, ring, npr, kshift, first);
-            run_pieces((int32_t)run.x, (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
+            run_pieces((int32_t)(run.x & 0xffffu), (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
                        [&](int32_t tile, int32_t lo, int32_t len) {
                 const uint2 v = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
                 const unsigned int pi = atomicAdd(&L.npieces, 1u);
@@ -821,7 +821,7 @@ int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *
 /* ------------------------------------------------------------------ */
 
 #define FSEG 256                   /* segments staged per batch */
-#define FB_LDS 256                 /* deferred pixels buffered in LDS */
+#define FB_LDS 128                 /* deferred pixels buffered in LDS */
 #define PMAX 4096                  /* pixels per batch (>= the widest tile) */
 
 /* the part of a batch both painters share: segment starts and the pixel
@@ -833,26 +833,33 @@ struct BatchLDS {
     uint32_t nb;
 };
 
-/* Geometry of tile t: ring, first in-ring pixel ip0, width, first map pixel
- * of the ring, and the pixel-centre z, sin(theta) and phi parameters. */
-struct TileGeo {
-    int32_t ring, ip0, width;
-    int64_t first;
-    double zr, sth, off, den;
-};
-
-__device__ static inline TileGeo tile_geo(const TileArgs &A, int64_t t, int W)
+__global__ void __launch_bounds__(256) k_tile_desc(int32_t nside, int32_t log2w, int64_t n_tiles, const int32_t *tile_ring,
+                                                   const int32_t *tile_base, TileDesc *desc)
 {
-    TileGeo g;
-    g.ring = A.tile_ring[t];
-    const int32_t col = (int32_t)(t - A.tile_base[g.ring - 1]);
-    int32_t npr, kshift;
-    ring_info(A.nside, g.ring, npr, kshift, g.first);
-    g.ip0 = col * W;
-    g.width = npr - g.ip0 < W ? npr - g.ip0 : W;
-    ring_zsth(A.nside, g.ring, g.zr, g.sth);
-    ring_phi_param(A.nside, g.ring, g.off, g.den);
-    return g;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_tiles; t += (int64_t)gridDim.x * blockDim.x) {
+        TileDesc d;
+        d.ring = tile_ring[t];
+        const int32_t W = 1 << log2w;
+        const int32_t col = (int32_t)(t - tile_base[d.ring - 1]);
+        int32_t npr, kshift;
+        ring_info(nside, d.ring, npr, kshift, d.first);
+        d.ip0 = col * W;
+        d.width = npr - d.ip0 < W ? npr - d.ip0 : W;
+        ring_zsth(nside, d.ring, d.zr, d.sth);
+        ring_phi_param(nside, d.ring, d.off, d.den);
+        d.kphi = HPX_HALFPI / d.den;
+        d.pad = 0;
+        desc[t] = d;
+    }
+}
+
+void p2m_launch_tile_desc(int32_t nside, int32_t log2w, int64_t n_tiles, const int32_t *tile_ring,
+                          const int32_t *tile_base, TileDesc *desc, hipStream_t s)
+{
+    if (n_tiles <= 0) return;
+    int64_t g = (n_tiles + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_tile_desc, dim3((unsigned)g), dim3(256), 0, s, nside, log2w, n_tiles, tile_ring, tile_base, desc);
 }
 
 /* XCD-aware tile order: workgroup b runs on XCD b % 8; the workgroups of
@@ -955,12 +962,12 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
     int64_t tbeg, tend, tstep;
     tile_range(A, tbeg, tend, tstep);
     for (int64_t t = tbeg; t < tend; t += tstep) {
-        const TileGeo G = tile_geo(A, t, W);
+        const TileDesc G = A.desc[t];
+        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
-        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
         for (uint64_t b0 = sbeg; b0 < send;) {
             __syncthreads();
             int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
@@ -1033,12 +1040,11 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
 struct FastSeg {
     BatchLDS b;
     double dphi0[FSEG];            /* phi offset of the ring's pixel 0 from the halo centre */
-    unsigned long long hh[FSEG];
     uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
     uint32_t lo[FSEG];
-    float A[FSEG], B[FSEG], thr_hi[FSEG], thr_lo[FSEG], chi2[FSEG], vh[FSEG];
-    float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];
+    float A[FSEG], B[FSEG], thr_hi[FSEG], chi2[FSEG];
+    float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];   /* corner weights times the velocity factor */
     int32_t colbase[FSEG];
     uint32_t nfb;
 };
@@ -1089,9 +1095,12 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     uint32_t *lcount = (uint32_t *)(smem_raw + acc_bytes + (A.count ? (size_t)W * 8 : 0));
     const size_t mem_bytes = A.count ? (size_t)W * 12 : 0;
     FastSeg *sl = (FastSeg *)(smem_raw + ((acc_bytes + mem_bytes + 15) & ~(size_t)15));
-    float *invn = (float *)(sl + 1);
+    unsigned long long *hh = (unsigned long long *)(sl + 1);   /* membership hashes, if counted */
+    float *invn = (float *)(hh + (A.count ? FSEG : 0));
     __shared__ unsigned long long lds16[16];
     __shared__ unsigned long long fb_base;
+    /* inner edge of the deferral band, relative to the outer one */
+    const float lo_band = (float)((1.0 - (double)A.band_c) / (1.0 + (double)A.band_c));
 
     const int nrt = A.g.nrt;
     const int64_t dmh = nrt, dch = (int64_t)nrt * A.g.nmht;
@@ -1107,13 +1116,13 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     int64_t tbeg, tend, tstep;
     tile_range(A, tbeg, tend, tstep);
     for (int64_t t = tbeg; t < tend; t += tstep) {
-        const TileGeo G = tile_geo(A, t, W);
-        const double kphi = HPX_HALFPI / G.den;
+        const TileDesc G = A.desc[t];
+        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
+        const double kphi = G.kphi;
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
-        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
         for (uint64_t b0 = sbeg; b0 < send;) {
             __syncthreads();
             int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
@@ -1129,17 +1138,15 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 sl->A[tid] = (float)(dz * dz + ds * ds);
                 sl->B[tid] = (float)((4.0 * G.sth) * h.s0);
                 sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
-                sl->thr_lo[tid] = (float)(h.thr * (1.0 - (double)A.band_c));
                 sl->chi2[tid] = h.chi2;
-                sl->vh[tid] = h.vh;
-                const float fc = h.fc, fm = h.fm;
-                sl->w00[tid] = (1.0f - fc) * (1.0f - fm);
-                sl->w10[tid] = fc * (1.0f - fm);
-                sl->w01[tid] = (1.0f - fc) * fm;
-                sl->w11[tid] = fc * fm;
+                const float fc = h.fc, fm = h.fm, vh = h.vh;
+                sl->w00[tid] = vh * ((1.0f - fc) * (1.0f - fm));
+                sl->w10[tid] = vh * (fc * (1.0f - fm));
+                sl->w01[tid] = vh * ((1.0f - fc) * fm);
+                sl->w11[tid] = vh * (fc * fm);
                 sl->colbase[tid] = h.colbase;
                 sl->dphi0[tid] = G.off * kphi - h.phi0;
-                sl->hh[tid] = h.hash;
+                if (A.count) hh[tid] = h.hash;
             }
             uint32_t P;
             const int nb = batch_prepare(&sl->b, len, nstage, P);
@@ -1155,7 +1162,8 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 if (x > HPX_HALFPI) x -= HPX_PI;
                 const float s2 = sin2f((float)x);
                 const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
-                if (!(c2 < sl->thr_hi[s])) continue;              /* outside the disc */
+                const float thr_hi = sl->thr_hi[s];
+                if (!(c2 < thr_hi)) continue;                      /* outside the disc */
                 const float h2 = 0.25f * c2;
                 const float dd = 1.0f - 2.0f * h2;
                 const float rd = __builtin_amdgcn_rcpf(dd);
@@ -1165,7 +1173,7 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 const float fl = __builtin_floorf(u);
                 const float fu = u - fl;
                 const bool zero = u > A.u_max + A.band;           /* r > rmaxt: value 0 */
-                const bool defer = c2 > sl->thr_lo[s] || !(dd > 1e-3f) ||
+                const bool defer = c2 > thr_hi * lo_band || !(dd > 1e-3f) ||
                                    (!zero && ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
                                               (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))));
                 if (defer) {
@@ -1186,7 +1194,6 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                                         : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 const int64_t c0 = (int64_t)sl->colbase[s] + (ir - 1);
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
-                const float vh = zero ? 0.0f : sl->vh[s];
                 n_upd++;
                 for (int m = 0; m < NM; ++m) {
                     const float *T = A.tab_prof[m];
@@ -1198,11 +1205,11 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                     acc4 = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc4);
                     acc4 = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc4);
                     acc4 = __builtin_fmaf(w11, __builtin_fmaf(fr, d1 - d0, d0), acc4);
-                    atomicAdd(&acc[m * W + q], (double)(vh * acc4));
+                    atomicAdd(&acc[m * W + q], zero ? 0.0 : (double)acc4);
                 }
                 if (A.count) {
                     atomicAdd(&lcount[q], 1u);
-                    atomicAdd(&lhash[q], sl->hh[s]);
+                    atomicAdd(&lhash[q], hh[s]);
                 }
             }
         }
@@ -1302,7 +1309,7 @@ size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int n
     size_t b = (size_t)n_maps * W * 8;
     if (membership) b += W * 12;
     b = (b + 15) & ~(size_t)15;
-    if (mode == 2) b += sizeof(FastSeg) + sizeof(float) * (size_t)(nrt + 2);
+    if (mode == 2) b += sizeof(FastSeg) + (membership ? 8 * FSEG : 0) + sizeof(float) * (size_t)(nrt + 2);
     else b += sizeof(ExactSeg);
     return b;
 }
