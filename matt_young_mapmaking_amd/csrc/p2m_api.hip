@@ -92,6 +92,7 @@ struct p2m_ctx {
     std::vector<int32_t> tile_base;
     int32_t *d_tile_base = nullptr;
     int32_t *d_tile_ring = nullptr;
+    int n_cu = 0;                   /* compute units of the device */
     TileDesc *d_tile_desc = nullptr;
     double *d_ring_z = nullptr;     /* ring2z of every ring */
     uint32_t *d_group = nullptr;    /* halo of every 64th (halo, ring) item */
@@ -235,6 +236,11 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
     HIPCHK(hipSetDevice(cfg->device));
     p2m_ctx *c = new p2m_ctx();
     c->cfg = *cfg;
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess) c->n_cu = ncu;
+        else (void)hipGetLastError();
+    }
     if (c->cfg.chunk_halos <= 0) c->cfg.chunk_halos = 1 << 24;
     if (c->cfg.chunk_halos > (1LL << 30)) c->cfg.chunk_halos = 1LL << 30;   /* chunk indices are 32-bit */
     memset(&c->st, 0, sizeof(c->st));
@@ -674,8 +680,17 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
             t.tab_flat[m] = c->d_flat[m]; t.tab_prof[m] = c->d_prof[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
         }
         t.count = c->d_count; t.hash = c->d_hash; t.counters = pc;
-        /* bounded persistent grid: each workgroup strides over tiles */
+        /* persistent grid of exactly the resident workgroups (a multiple
+         * of 8 for the XCD-aware tile order): no partial last round */
         int grid = (int)std::min<int64_t>(c->n_tiles, 8192);
+        {
+            const size_t lds = p2m_tile_lds_bytes(mode, c->cfg.n_maps, c->log2w, c->d_count != nullptr, c->g.nrt);
+            const int occ = p2m_tile_occupancy(mode, c->cfg.n_maps, c->wg, lds);
+            if (occ > 0 && c->n_cu > 0) {
+                const int64_t res = ((int64_t)occ * c->n_cu) & ~(int64_t)7;
+                if (res >= 8) grid = (int)std::min<int64_t>(c->n_tiles, res);
+            }
+        }
         if (p2m_launch_tile_paint(t, mode, c->cfg.n_maps, grid, c->wg, c->stream)) return P2M_E_INVALID;
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[7], c->stream));

@@ -154,6 +154,7 @@ int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin,
                    hipStream_t s);
 int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int nrt);
+int p2m_tile_occupancy(int mode, int n_maps, int wg, size_t lds);
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);
 int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s);
 void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double mean, double *out,

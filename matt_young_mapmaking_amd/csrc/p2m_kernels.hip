@@ -820,6 +820,16 @@ int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *
 /* tile painters: common parts                                          */
 /* ------------------------------------------------------------------ */
 
+/* Workgroup barrier for LDS traffic only: waits for this wave's LDS
+ * operations, not for its outstanding global loads and stores, then
+ * s_barrier.  (__syncthreads also drains global memory, which exposes the
+ * latency of the map stores at every tile boundary.) */
+__device__ static inline void lds_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(0xc07f);      /* lgkmcnt(0); vmcnt, expcnt unconstrained */
+    __builtin_amdgcn_s_barrier();
+}
+
 #define FSEG 256                   /* segments staged per batch */
 #define FB_LDS 128                 /* deferred pixels buffered in LDS */
 #define PMAX 4096                  /* pixels per batch (>= the widest tile) */
@@ -887,18 +897,18 @@ __device__ static inline int batch_prepare(BatchLDS *bl, uint32_t len, int nstag
         if (lane >= o) v += u;
     }
     if (lane == 63) bl->wsum[wv] = v;
-    __syncthreads();
+    lds_barrier();
     uint32_t add = 0;
     for (int k = 0; k < wv; ++k) add += bl->wsum[k];
     v += add;
     if (tid < nstage) bl->pre[tid + 1] = v;
     if (tid == 0) bl->pre[0] = 0;
-    __syncthreads();
+    lds_barrier();
     if (tid < nstage) {
         const uint32_t e = bl->pre[tid + 1];
         if (e <= PMAX && (tid + 1 == nstage || bl->pre[tid + 2] > PMAX)) bl->nb = (uint32_t)(tid + 1);
     }
-    __syncthreads();
+    lds_barrier();
     const int nb = (int)bl->nb;
     P = bl->pre[nb];
     /* each thread takes a run of whole words, finds the segment of its first
@@ -925,7 +935,7 @@ __device__ static inline int batch_prepare(BatchLDS *bl, uint32_t len, int nstag
             bl->seg_of[w] = o;
         }
     }
-    __syncthreads();
+    lds_barrier();
     return nb;
 }
 
@@ -969,7 +979,7 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
         for (uint64_t b0 = sbeg; b0 < send;) {
-            __syncthreads();
+            lds_barrier();
             int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
             if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
@@ -1001,7 +1011,7 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         const int64_t pix0 = G.first + G.ip0;
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) {
@@ -1016,7 +1026,7 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
                 *dh = A.accumulate ? *dh + lhash[q] : lhash[q];
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
     block_count(&A.counters[1], n_upd, lds16);
 }
@@ -1111,7 +1121,7 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
     uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
     for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
     if (tid == 0) sl->nfb = 0;
-    __syncthreads();
+    lds_barrier();
 
     int64_t tbeg, tend, tstep;
     tile_range(A, tbeg, tend, tstep);
@@ -1124,7 +1134,7 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
         for (uint64_t b0 = sbeg; b0 < send;) {
-            __syncthreads();
+            lds_barrier();
             int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
             if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
@@ -1213,7 +1223,7 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         const int64_t pix0 = G.first + G.ip0;
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) {
@@ -1232,19 +1242,19 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
         if (sl->nfb >= FB_LDS / 2) {
             const uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
             if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
-            __syncthreads();
+            lds_barrier();
             for (uint32_t k = tid; k < nf; k += blockDim.x)
                 if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
-            __syncthreads();
+            lds_barrier();
             if (tid == 0) sl->nfb = 0;
         }
-        __syncthreads();
+        lds_barrier();
     }
     {
         const uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
         if (nf) {
             if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
-            __syncthreads();
+            lds_barrier();
             for (uint32_t k = tid; k < nf; k += blockDim.x)
                 if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
         }
@@ -1312,6 +1322,25 @@ size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int n
     if (mode == 2) b += sizeof(FastSeg) + (membership ? 8 * FSEG : 0) + sizeof(float) * (size_t)(nrt + 2);
     else b += sizeof(ExactSeg);
     return b;
+}
+
+/* workgroups of the tile painter resident per CU at this shape (0 on error) */
+int p2m_tile_occupancy(int mode, int n_maps, int wg, size_t lds)
+{
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define P2M_OC(K) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, K, wg, lds)
+    switch (mode * 8 + n_maps) {
+    case 9: P2M_OC(k_tile_exact<1>); break;   case 10: P2M_OC(k_tile_exact<2>); break;
+    case 11: P2M_OC(k_tile_exact<3>); break;  case 12: P2M_OC(k_tile_exact<4>); break;
+    case 13: P2M_OC(k_tile_exact<5>); break;  case 14: P2M_OC(k_tile_exact<6>); break;
+    case 17: P2M_OC(k_tile_fast<1>); break;   case 18: P2M_OC(k_tile_fast<2>); break;
+    case 19: P2M_OC(k_tile_fast<3>); break;   case 20: P2M_OC(k_tile_fast<4>); break;
+    case 21: P2M_OC(k_tile_fast<5>); break;   case 22: P2M_OC(k_tile_fast<6>); break;
+    default: break;
+    }
+#undef P2M_OC
+    return e == hipSuccess ? nb : 0;
 }
 
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s)
