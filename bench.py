@@ -214,8 +214,9 @@ def main():
         "fp64_tiled_halos_per_s": n / sec64,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (n / sec32) / plain["halos_per_s"] if plain else None,
-        "precision_note": "per-pixel profile and LDS accumulation in f32 with two-sum compensation; "
-                          "centre angles, redshift inversion, membership edge band and tile reduce in f64",
+        "precision_note": "per-pixel profile in f32, summed in f64 LDS tiles (compensated accumulation), "
+                          "one f32 rounding per map pixel; centre angles, azimuth offsets, redshift inversion "
+                          "and the membership edge band in f64",
         "phase_ms": {k: float(np.mean([x[k] for x in st32]))
                      for k in ("ms_decode", "ms_bin", "ms_paint", "ms_tile", "ms_finish")},
         "host_memory_halos_per_s": n / sec_host,

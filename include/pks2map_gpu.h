@@ -39,8 +39,9 @@ extern "C" {
 /* precision / algorithm of the painter */
 #define P2M_PREC_PLAIN_FP64  0  /* one work item per halo, FP64 everything, FP64 global atomics (baseline) */
 #define P2M_PREC_FP64        1  /* tiled painter, FP64 geometry + FP64 profile, FP64 LDS tiles */
-#define P2M_PREC_FP32_COMP   2  /* tiled painter, FP64 geometry, FP32 profile with FP64 fallback band,
-                                   FP32 two-sum LDS tiles, FP64 tile-to-map reduce */
+#define P2M_PREC_FP32_COMP   2  /* tiled painter, FP64 geometry, FP32 profile with FP64 fallback band;
+                                   the FP32 values are summed in FP64 LDS tiles (error-compensated
+                                   accumulation) and each map pixel is rounded to FP32 once */
 
 /* flags */
 #define P2M_FLAG_MEMBERSHIP  1  /* keep per-pixel contribution count and halo-hash sum */
