@@ -81,7 +81,10 @@ typedef struct {
     int64_t n_ring_gaps;    /* empty rings between non-empty rings (quirk Q15); expect 0 */
     int64_t n_radius_err;   /* halos with disc radius outside [0, pi] (skipped) */
     double  cut_low, cut_high;
-    double  ms_total, ms_decode, ms_bin, ms_paint, ms_finish;  /* last p2m_paint* call */
+    /* device time of the last p2m_paint* call: total, decode, binning (spatial
+     * sort to segment lists), healpix painting, flat-sky painting (the tiled
+     * modes run it inside the binning's host wait; ms_bin excludes it) */
+    double  ms_total, ms_decode, ms_bin, ms_paint, ms_finish;
     int32_t tile_w, tile_wg;  /* tile shape used (autotuned or set) */
     double  ms_tile;          /* the tile painter kernel alone, within ms_paint (0 for plain FP64) */
 } p2m_stats;

@@ -43,7 +43,7 @@ namespace {
 const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
 
 struct Ctr {                 /* layout of one shard row of the device counter block */
-    enum { DEC = 1, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, TOTAL = P2M_CTR_STRIDE,
+    enum { DEC = 1, ITEMS = 4, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, TOTAL = P2M_CTR_STRIDE,
            ALL = P2M_CTR_STRIDE * P2M_SHARDS };
 };
 
@@ -52,7 +52,8 @@ struct Ctr {                 /* layout of one shard row of the device counter bl
 struct p2m_ctx {
     p2m_config cfg;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8];
+    hipEvent_t ev[10];
+    unsigned long long *h_pin = nullptr;   /* pinned host words for mid-call readbacks */
     bool have_run = false;
     bool have_table[P2M_MAXMAPS] = {false};
     p2m_run run;
@@ -245,7 +246,11 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
     if (c->cfg.chunk_halos > (1LL << 30)) c->cfg.chunk_halos = 1LL << 30;   /* chunk indices are 32-bit */
     memset(&c->st, 0, sizeof(c->st));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return P2M_E_HIP; }
-    for (int i = 0; i < 8; ++i) (void)hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 10; ++i) (void)hipEventCreate(&c->ev[i]);
+    if (hipHostMalloc((void **)&c->h_pin, sizeof(unsigned long long) * (Ctr::ALL + 8), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->h_pin = nullptr;
+    }
     c->hpnpix = cfg->nside > 0 ? 12LL * cfg->nside * cfg->nside : 0;
     c->fsnpix = (int64_t)cfg->npix_fs * cfg->npix_fs;
     int rc = 0;
@@ -289,7 +294,8 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_tile_ring); dfree(c->d_tile_desc); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
     dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
-    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 10; ++i) (void)hipEventDestroy(c->ev[i]);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -559,9 +565,11 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
     return hipGetLastError() == hipSuccess ? 0 : P2M_E_HIP;
 }
 
-static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat);
+static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long long n_items);
 
-static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tune)
+static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat);
+
+static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long long n_items, bool allow_tune)
 {
     const int mode = c->cfg.precision;
     unsigned long long *pc = c->d_ctr + Ctr::PAINT;
@@ -591,7 +599,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
     }
     if (c->cfg.nside > 0) {
         if (!c->tuned) {
-            if (allow_tune) { int rc = autotune(c, n_halo, n_flat); if (rc) return rc; }
+            if (allow_tune) { int rc = autotune(c, n_halo, n_flat, n_items); if (rc) return rc; }
             else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
         }
         /* painted halos in spatial order (keys written by decode):
@@ -620,9 +628,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         /* (halo, ring) item space */
         p2m_launch_ring_counts(b, c->stream);
         { int rc = scan_counts(c, c->d_nr, c->d_ring_off, n_halo + 1); if (rc) return rc; }
-        unsigned long long n_items = 0;
-        HIPCHK(hipMemcpyAsync(&n_items, c->d_ring_off + n_halo, sizeof(n_items), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        /* n_items (= ring_off[n_halo]) was counted by the decode */
         if ((int64_t)n_items > c->runs_cap || !c->d_runs) {
             dfree(c->d_runs); c->d_runs = nullptr; c->runs_cap = 0;
             int64_t cap = (int64_t)(n_items * 1.25) + 1024;
@@ -640,12 +646,21 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         p2m_launch_fill_u32(c->d_tile_fill, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_seg_count(b, c->stream);
         { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
+        /* segment and listed-pixel counts back to the host; the flat-sky
+         * painter (independent of the binning) is queued behind the copies
+         * so the device works while the host waits for them */
         unsigned long long nseg = 0, nlisted = 0;
-        HIPCHK(hipMemcpyAsync(&nseg, c->d_tile_off + c->n_tiles, sizeof(nseg), hipMemcpyDeviceToHost, c->stream));
         {
-            unsigned long long hc[Ctr::TOTAL];
-            int rc = read_counters(c, hc); if (rc) return rc;
-            nlisted = hc[Ctr::PAINT + 0];
+            unsigned long long hbuf[Ctr::ALL + 1];
+            unsigned long long *hb = c->h_pin ? c->h_pin : hbuf;
+            HIPCHK(hipMemcpyAsync(hb, c->d_ctr, sizeof(unsigned long long) * Ctr::ALL, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(hb + Ctr::ALL, c->d_tile_off + c->n_tiles, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipEventRecord(c->ev[6], c->stream));
+            { int rc = launch_flat(c, pa, n_flat); if (rc) return rc; }
+            HIPCHK(hipEventSynchronize(c->ev[6]));
+            nseg = hb[Ctr::ALL];
+            for (int sh = 0; sh < P2M_SHARDS; ++sh) nlisted += hb[sh * Ctr::TOTAL + Ctr::PAINT + 0];
         }
         if (mode == P2M_PREC_FP32_COMP) {
             /* deferred FP64 pixels: ~5e-4 of the listed pixels in practice */
@@ -704,10 +719,21 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
             f.count = c->d_count; f.hash = c->d_hash; f.counters = pc;
             if (p2m_launch_fallback(f, (int64_t)(nlisted / 256) + 1024, c->stream)) return P2M_E_HIP;
         }
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
     } else {
         HIPCHK(hipEventRecord(c->ev[3], c->stream));
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        { int rc = launch_flat(c, pa, n_flat); if (rc) return rc; }
     }
-    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    c->hp_zero = false;
+    return 0;
+}
+
+/* flat-sky painter over the flattened box pixels, between events 8 and 9 */
+static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat)
+{
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
     if (c->cfg.npix_fs > 0 && n_flat > 0) {
         if (n_flat + 1 > c->fs_cap || !c->d_fs_cnt) {
             dfree(c->d_fs_cnt); dfree(c->d_fs_off); c->d_fs_cnt = nullptr; c->d_fs_off = nullptr; c->fs_cap = 0;
@@ -722,15 +748,13 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, bool allow_tu
         p2m_launch_fs_paint(pa, c->stream);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    c->hp_zero = false;
+    HIPCHK(hipEventRecord(c->ev[9], c->stream));
     return 0;
 }
 
 static int finish_stats(p2m_ctx *c)
 {
     unsigned long long h[Ctr::TOTAL], raw[Ctr::ALL];
-    HIPCHK(hipEventRecord(c->ev[6], c->stream));
     int rc = read_counters(c, h, raw);
     if (rc) return rc;
     p2m_stats &s = c->st;
@@ -757,9 +781,15 @@ static int finish_stats(p2m_ctx *c)
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); s.ms_total = ms;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); s.ms_decode = ms;
-    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]); s.ms_bin = ms;
+    float ms_fs = 0;
+    if (c->cfg.precision != P2M_PREC_PLAIN_FP64) (void)hipEventElapsedTime(&ms_fs, c->ev[8], c->ev[9]);
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+    /* the flat-sky painter of the tiled modes runs inside the binning's
+     * host-synchronisation gap: its time is reported as the finish phase */
+    s.ms_bin = (c->cfg.nside > 0 && c->cfg.precision != P2M_PREC_PLAIN_FP64) ? std::max(0.0f, ms - ms_fs) : ms;
     (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); s.ms_paint = ms;
-    (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]); s.ms_finish = ms;
+    if (c->cfg.precision == P2M_PREC_PLAIN_FP64) { (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]); s.ms_finish = ms; }
+    else s.ms_finish = ms_fs;
     s.ms_tile = 0.0;
     if (c->cfg.precision != P2M_PREC_PLAIN_FP64 && c->cfg.nside > 0) {
         (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[7]); s.ms_tile = ms;
@@ -809,7 +839,7 @@ static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n,
     if (rc) return rc;
     int64_t n_halo = (int64_t)std::min<unsigned long long>(hc[Ctr::N_CAND], (unsigned long long)c->halo_cap);
     int64_t n_flat = (int64_t)std::min<unsigned long long>(hc[Ctr::N_FLAT], (unsigned long long)c->halo_cap);
-    rc = paint_halos(c, n_halo, n_flat, allow_tune);
+    rc = paint_halos(c, n_halo, n_flat, hc[Ctr::ITEMS], allow_tune);
     if (rc) return rc;
     c->hp_valid = true;
     c->st.n_in = n;
@@ -889,7 +919,7 @@ int p2m_fetch_membership(p2m_ctx *c, uint32_t *hc, uint64_t *hh, uint32_t *fc, u
 /* ------------------------------------------------------------------ autotuner */
 /* Times the binning + tile paint for each candidate tile shape on the
  * halos already decoded, keeps the fastest, and restores the maps. */
-static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat)
+static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long long n_items)
 {
     static const int cand[][2] = {{9, 256}, {10, 256}, {11, 256}, {11, 128}, {12, 256}};
     int mode_ld = c->cfg.precision == P2M_PREC_FP64 ? 1 : 2;
@@ -915,7 +945,7 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat)
         int rc = build_tiles(c, cd[0]); if (rc) return rc;
         c->wg = cd[1]; c->tuned = true;
         c->hp_zero = true;
-        rc = paint_halos(c, n_halo, n_flat, false);
+        rc = paint_halos(c, n_halo, n_flat, n_items, false);
         if (rc == P2M_E_INVALID) { (void)p2m_clear(c); continue; }
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->stream));

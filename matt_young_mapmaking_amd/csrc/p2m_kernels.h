@@ -36,7 +36,7 @@ struct DecodeArgs {
     uint32_t *keys, *vals;       /* spatial sort key and slot of each record (tiled painters) */
     int32_t nphi;                /* azimuth bins of the key */
     uint32_t key_max;            /* key of inert records (sorted last) */
-    unsigned long long *counters;/* sharded: [0] passed radial cut, [2] radius errors */
+    unsigned long long *counters;/* sharded: [0] passed radial cut, [2] radius errors, [3] (halo, ring) items */
 };
 
 struct PaintArgs {

@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
     const CosmoDev &c = A.c;
     const TabGeom &g = A.g;
     __shared__ unsigned long long lds16[16];
-    unsigned long long tot_rad = 0;
+    unsigned long long tot_rad = 0, tot_rings = 0;
     const unsigned long long kbeg = *A.n_cand0, kend = *A.n_cand;
     for (unsigned long long k0 = kbeg + blockIdx.x * (unsigned long long)blockDim.x; k0 < kend;
          k0 += (unsigned long long)gridDim.x * blockDim.x) {
@@ -266,6 +266,8 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
             }
         }
         const bool flat = k < kend && painted && h.flat;
+        /* (halo, ring) items of this record, as the binning counts them */
+        if (k < kend && k < (unsigned long long)A.cap) tot_rings += h.irmax >= h.irmin ? (unsigned long long)(h.irmax - h.irmin + 1) : 1ULL;
         if (k < kend && k < (unsigned long long)A.cap) {
             A.out[k] = h;
             if (A.keys) {
@@ -287,6 +289,7 @@ __global__ void __launch_bounds__(256) k_setup(DecodeArgs A)
         if (flat) A.flat_idx[fs] = (uint32_t)k;
     }
     block_count(&A.counters[2], tot_rad, lds16);
+    block_count(&A.counters[3], tot_rings, lds16);
 }
 
 /* n_cand0 <- n_cand: the first candidate slot of the next chunk */
