@@ -42,8 +42,10 @@ namespace {
 
 const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
 
-struct Ctr {                 /* layout of one shard row of the device counter block */
-    enum { DEC = 1, ITEMS = 4, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, TOTAL = P2M_CTR_STRIDE,
+struct Ctr {                 /* layout of one shard row of the device counter block; OVF is
+                                the paint counter (PAINT + OVF, shard 0 only) of binning
+                                overflow-list entries */
+    enum { DEC = 1, ITEMS = 4, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, OVF = 7, TOTAL = P2M_CTR_STRIDE,
            ALL = P2M_CTR_STRIDE * P2M_SHARDS };
 };
 
@@ -84,8 +86,10 @@ struct p2m_ctx {
     uint32_t *d_flat_idx = nullptr;
     uint32_t *d_nr = nullptr;
     unsigned long long *d_ring_off = nullptr;
-    uint2 *d_runs = nullptr;
+    uint2 *d_runs = nullptr, *d_pos = nullptr;   /* per (halo, ring) item: run, piece slots */
     int64_t runs_cap = 0;
+    uint4 *d_ovf = nullptr;                       /* pieces beyond an item's second */
+    int64_t ovf_cap = 0;
     int64_t halo_cap = 0;
     unsigned long long *d_ctr = nullptr;
     int log2w = 0, wg = 0;
@@ -103,7 +107,7 @@ struct p2m_ctx {
     int64_t fs_cap = 0;
     int64_t n_tiles = 0;
     int n_rings = 0;
-    uint32_t *d_tile_cnt = nullptr, *d_tile_fill = nullptr;
+    uint32_t *d_tile_cnt = nullptr;
     unsigned long long *d_tile_off = nullptr;
     void *d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
@@ -174,16 +178,15 @@ static int build_tiles(p2m_ctx *c, int log2w)
         const int64_t e = r < c->n_rings ? c->tile_base[r] : acc;
         for (int64_t t = c->tile_base[r - 1]; t < e; ++t) tile_ring[(size_t)t] = r;
     }
-    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_tile_ring);
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_off); dfree(c->d_tile_ring);
     dfree(c->d_tile_desc);
-    c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_fill = nullptr; c->d_tile_off = nullptr;
+    c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_off = nullptr;
     c->d_tile_ring = nullptr; c->d_tile_desc = nullptr;
     int rc = 0;
     rc |= dmalloc((void **)&c->d_tile_ring, sizeof(int32_t) * acc);
     rc |= dmalloc((void **)&c->d_tile_desc, sizeof(TileDesc) * acc);
     rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
     rc |= dmalloc((void **)&c->d_tile_cnt, sizeof(uint32_t) * (c->n_tiles + 1));
-    rc |= dmalloc((void **)&c->d_tile_fill, sizeof(uint32_t) * (c->n_tiles + 1));
     rc |= dmalloc((void **)&c->d_tile_off, sizeof(unsigned long long) * (c->n_tiles + 1));
     if (rc) return P2M_E_NOMEM;
     HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
@@ -289,8 +292,8 @@ void p2m_destroy(p2m_ctx *c)
     }
     dfree(c->d_count); dfree(c->d_hash); dfree(c->d_fs_count); dfree(c->d_fs_hash);
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
-    dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_ctr);
-    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_fill); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
+    dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_pos); dfree(c->d_ovf); dfree(c->d_ctr);
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
     dfree(c->d_tile_ring); dfree(c->d_tile_desc); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
     dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
@@ -623,17 +626,23 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         memset(&b, 0, sizeof(b));
         b.halos = c->d_bin; b.n_halo = n_halo; b.nside = c->cfg.nside; b.log2w = c->log2w;
         b.nr = c->d_nr; b.ring_off = c->d_ring_off;
-        b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt; b.tile_fill = c->d_tile_fill;
+        b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt;
         b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc; b.ring_z = c->d_ring_z;
         /* (halo, ring) item space */
         p2m_launch_ring_counts(b, c->stream);
         { int rc = scan_counts(c, c->d_nr, c->d_ring_off, n_halo + 1); if (rc) return rc; }
         /* n_items (= ring_off[n_halo]) was counted by the decode */
-        if ((int64_t)n_items > c->runs_cap || !c->d_runs) {
-            dfree(c->d_runs); c->d_runs = nullptr; c->runs_cap = 0;
+        if ((int64_t)n_items > c->runs_cap || !c->d_runs || !c->d_pos) {
+            dfree(c->d_runs); dfree(c->d_pos); c->d_runs = nullptr; c->d_pos = nullptr; c->runs_cap = 0;
             int64_t cap = (int64_t)(n_items * 1.25) + 1024;
             if (dmalloc((void **)&c->d_runs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
+            if (dmalloc((void **)&c->d_pos, sizeof(uint2) * cap)) return P2M_E_NOMEM;
             c->runs_cap = cap;
+        }
+        if (!c->d_ovf) {
+            const int64_t cap = 1 << 16;
+            if (dmalloc((void **)&c->d_ovf, sizeof(uint4) * cap)) return P2M_E_NOMEM;
+            c->ovf_cap = cap;
         }
         if ((int64_t)(n_items / 64 + 2) > c->group_cap || !c->d_group) {
             dfree(c->d_group); c->d_group = nullptr; c->group_cap = 0;
@@ -641,9 +650,9 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             if (dmalloc((void **)&c->d_group, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
             c->group_cap = cap;
         }
-        b.n_items = n_items; b.runs = c->d_runs; b.group_halo = c->d_group;
+        b.n_items = n_items; b.runs = c->d_runs; b.pos = c->d_pos; b.group_halo = c->d_group;
+        b.ovf = c->d_ovf; b.ovf_cap = c->ovf_cap; b.n_ovf = pc + Ctr::OVF;
         p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
-        p2m_launch_fill_u32(c->d_tile_fill, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_seg_count(b, c->stream);
         { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
         /* segment and listed-pixel counts back to the host; the flat-sky
@@ -659,6 +668,26 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             HIPCHK(hipEventRecord(c->ev[6], c->stream));
             { int rc = launch_flat(c, pa, n_flat); if (rc) return rc; }
             HIPCHK(hipEventSynchronize(c->ev[6]));
+            const unsigned long long n_ovf = hb[Ctr::PAINT + Ctr::OVF];
+            if ((int64_t)n_ovf > c->ovf_cap) {
+                /* pieces beyond the overflow list (runs spanning three or
+                 * more tiles): grow it and count again */
+                dfree(c->d_ovf); c->d_ovf = nullptr; c->ovf_cap = 0;
+                const int64_t cap = (int64_t)(n_ovf * 1.25) + 1024;
+                if (dmalloc((void **)&c->d_ovf, sizeof(uint4) * cap)) return P2M_E_NOMEM;
+                c->ovf_cap = cap;
+                b.ovf = c->d_ovf; b.ovf_cap = cap;
+                p2m_launch_clear_ctr(pc, 0, 1, c->stream);
+                p2m_launch_clear_ctr(pc, 4, 1, c->stream);
+                p2m_launch_clear_ctr(pc, Ctr::OVF, 1, c->stream);
+                p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
+                p2m_launch_seg_count(b, c->stream);
+                { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
+                HIPCHK(hipMemcpyAsync(hb, c->d_ctr, sizeof(unsigned long long) * Ctr::ALL, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipMemcpyAsync(hb + Ctr::ALL, c->d_tile_off + c->n_tiles, sizeof(unsigned long long),
+                                      hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+            }
             nseg = hb[Ctr::ALL];
             for (int sh = 0; sh < P2M_SHARDS; ++sh) nlisted += hb[sh * Ctr::TOTAL + Ctr::PAINT + 0];
         }

@@ -522,28 +522,26 @@ __global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
     }
 }
 
-/* Tile counts and segment slots are aggregated per workgroup in an LDS
- * hash table keyed by tile: a chunk of BIN_CH consecutive (halo, ring)
- * items touches few distinct tiles once the halos are in spatial order, so
- * one global atomic per (chunk, tile) replaces one per segment.  A piece
- * whose tile finds no hash slot within BIN_PROBE probes, or that finds the
- * piece buffer full, takes the direct path (one global atomic). */
+/* Segment slots are assigned per workgroup through an LDS hash table keyed
+ * by tile: a chunk of BIN_CH consecutive (halo, ring) items touches few
+ * distinct tiles once the halos are in spatial order, so one global atomic
+ * per (chunk, tile) reserves the slots of all its pieces.  A piece whose
+ * tile finds no hash slot within BIN_PROBE probes takes one global atomic
+ * of its own. */
 #define BIN_CH 1024
 #define BIN_HASH 1024
 #define BIN_HASH_BITS 10
 #define BIN_PROBE 32
-#define BIN_PIECES 1280
 #define BIN_EMPTY 0xffffffffu
+#define BIN_DIRECT 0x80000000u
 
 struct BinHash {
     uint32_t key[BIN_HASH];
     uint32_t cnt[BIN_HASH];
-    unsigned int npieces;
 };
 
-struct BinLDS : BinHash {
-    uint2 val[BIN_PIECES];       /* segment (halo, lo<<16|len) */
-    uint32_t where[BIN_PIECES];  /* hash slot << 16 | rank within (chunk, tile); BIN_EMPTY: none */
+struct CountLDS : BinHash {
+    uint2 prec[BIN_CH];          /* per item of the chunk: its two pieces' slot codes */
 };
 
 __device__ static inline int bin_slot(BinHash *L, uint32_t tile)
@@ -560,7 +558,6 @@ __device__ static inline int bin_slot(BinHash *L, uint32_t tile)
 __device__ static inline void bin_reset(BinHash *L)
 {
     for (int k = threadIdx.x; k < BIN_HASH; k += blockDim.x) { L->key[k] = BIN_EMPTY; L->cnt[k] = 0; }
-    if (threadIdx.x == 0) L->npieces = 0;
 }
 
 /* the items [start, end) of one wave (start a multiple of 64), with their
@@ -610,10 +607,43 @@ __global__ void __launch_bounds__(256) k_group_heads(BinArgs B)
     }
 }
 
-/* per (halo, ring) item: the query_disc pixel run (stored for emit) and its tile counts */
+/* Per (halo, ring) item: the query_disc pixel run, its tile pieces and the
+ * slot of each piece within its tile's segment list.  A piece's slot code is
+ * (hash slot << 16 | rank within the chunk's pieces of that tile) until the
+ * chunk reserves its range in the tile, BIN_DIRECT | slot when reserved on
+ * its own, BIN_EMPTY for none.  Pieces beyond an item's second (runs
+ * spanning three or more tiles) are listed in ovf with their slot. */
+__device__ static inline uint32_t slot_of(const BinHash &L, uint32_t code)
+{
+    if (code == BIN_EMPTY) return 0u;
+    if (code & BIN_DIRECT) return code & ~BIN_DIRECT;
+    return L.cnt[code >> 16] + (code & 0xffffu);
+}
+
+/* XCD-aware split of n work units over the grid: workgroup b runs on XCD
+ * b % 8, and the workgroups of one XCD take one contiguous eighth of the
+ * units, so the units in flight on an XCD are spatial neighbours whose tile
+ * lists meet in its L2 (whole lines written before they leave it). */
+__device__ static inline void xcd_range(unsigned long long n, unsigned long long &beg, unsigned long long &end,
+                                        unsigned long long &step, unsigned long long lane_units, unsigned long long unit)
+{
+    const bool xcd = (gridDim.x % 8) == 0 && n >= 8;
+    const unsigned long long per_wg = lane_units;     /* units a workgroup takes per step */
+    if (xcd) {
+        const unsigned long long x = blockIdx.x % 8, l = blockIdx.x / 8;
+        beg = x * n / 8 + l * per_wg + unit;
+        end = (x + 1) * n / 8;
+        step = (unsigned long long)(gridDim.x / 8) * per_wg;
+    } else {
+        beg = (unsigned long long)blockIdx.x * per_wg + unit;
+        end = n;
+        step = (unsigned long long)gridDim.x * per_wg;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
 {
-    __shared__ BinHash L;
+    __shared__ CountLDS L;
     __shared__ unsigned long long lds16[16];
     unsigned long long n_list = 0, n_seg = 0;
     const unsigned long long nchunk = (B.n_items + BIN_CH - 1) / BIN_CH;
@@ -621,8 +651,11 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
     __syncthreads();
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const unsigned long long per = BIN_CH / nwv;
-    for (unsigned long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
-        unsigned long long a = ch * BIN_CH + wv * per, e = a + per;
+    unsigned long long cbeg, cend, cstep;
+    xcd_range(nchunk, cbeg, cend, cstep, 1, 0);
+    for (unsigned long long ch = cbeg; ch < cend; ch += cstep) {
+        const unsigned long long c0 = ch * BIN_CH;
+        unsigned long long a = c0 + wv * per, e = a + per;
         if (e > B.n_items) e = B.n_items;
         wave_range(B.ring_off, B.group_halo, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
             const BinHalo &h = B.halos[hi];
@@ -635,19 +668,40 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
             }
             /* (first pixel | ring << 16, count): both below 2^16 for nside <= 8192 */
             B.runs[item] = make_uint2((uint32_t)lo | ((uint32_t)ring << 16), (uint32_t)np);
+            uint32_t code[2] = {BIN_EMPTY, BIN_EMPTY};
             if (np > 0) {
                 n_list += (unsigned long long)np;
-                run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t, int32_t) {
-                    const int sl = bin_slot(&L, (uint32_t)tile);
-                    if (sl >= 0) atomicAdd(&L.cnt[sl], 1u);
-                    else atomicAdd(&B.tile_cnt[tile], 1u);
+                int j = 0;
+                run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t plo, int32_t len) {
+                    const int sl = j < 2 ? bin_slot(&L, (uint32_t)tile) : -1;
+                    if (sl >= 0) {
+                        code[j] = ((uint32_t)sl << 16) | atomicAdd(&L.cnt[sl], 1u);
+                    } else {
+                        const uint32_t slot = atomicAdd(&B.tile_cnt[tile], 1u);
+                        if (j < 2) {
+                            code[j] = BIN_DIRECT | slot;
+                        } else {
+                            const unsigned long long o = atomicAdd(B.n_ovf, 1ULL);
+                            if (o < (unsigned long long)B.ovf_cap)
+                                B.ovf[o] = make_uint4((uint32_t)tile, slot, (uint32_t)hi, ((uint32_t)plo << 16) | (uint32_t)len);
+                        }
+                    }
+                    ++j;
                     n_seg++;
                 });
             }
+            L.prec[item - c0] = make_uint2(code[0], code[1]);
         });
         __syncthreads();
+        /* reserve each (chunk, tile) range; cnt[] then holds its first slot */
         for (int k = threadIdx.x; k < BIN_HASH; k += blockDim.x)
-            if (L.key[k] != BIN_EMPTY) atomicAdd(&B.tile_cnt[L.key[k]], L.cnt[k]);
+            if (L.key[k] != BIN_EMPTY) L.cnt[k] = atomicAdd(&B.tile_cnt[L.key[k]], L.cnt[k]);
+        __syncthreads();
+        const unsigned long long nin = B.n_items - c0 < BIN_CH ? B.n_items - c0 : BIN_CH;
+        for (unsigned long long k = threadIdx.x; k < nin; k += blockDim.x) {
+            const uint2 cd = L.prec[k];
+            B.pos[c0 + k] = make_uint2(slot_of(L, cd.x), slot_of(L, cd.y));
+        }
         __syncthreads();
         bin_reset(&L);
         __syncthreads();
@@ -656,20 +710,18 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
     block_count(&B.counters[4], n_seg, lds16);
 }
 
+/* Scatter of every piece to its slot: segs[tile_off[tile] + slot]. */
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
-    __shared__ BinLDS L;
     __shared__ unsigned long long lds16[16];
     unsigned long long n_gaps = 0;
-    const unsigned long long nchunk = (B.n_items + BIN_CH - 1) / BIN_CH;
-    bin_reset(&L);
-    __syncthreads();
-    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const unsigned long long per = BIN_CH / nwv;
-    for (unsigned long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
-        unsigned long long a = ch * BIN_CH + wv * per, e = a + per;
-        if (e > B.n_items) e = B.n_items;
-        wave_range(B.ring_off, B.group_halo, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
+    const unsigned long long ngroups = (B.n_items + 63) / 64;
+    unsigned long long gbeg, gend, gstep;
+    xcd_range(ngroups, gbeg, gend, gstep, blockDim.x >> 6, threadIdx.x >> 6);
+    for (unsigned long long g = gbeg; g < gend; g += gstep) {
+        const unsigned long long c = g * 64;
+        const unsigned long long e = B.n_items - c < 64 ? B.n_items : c + 64;
+        wave_range(B.ring_off, B.group_halo, B.n_halo, c, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
             const uint2 run = B.runs[item];
             if (run.y == 0) {
                 /* an empty ring between non-empty rings of one disc, counted
@@ -682,46 +734,24 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
                 }
                 return;
             }
+            const uint2 ps = B.pos[item];
             const int32_t ring = (int32_t)(run.x >> 16);
             int32_t npr, kshift; int64_t first;
             ring_info(B.nside, ring, npr, kshift, first);
+            int j = 0;
             run_pieces((int32_t)(run.x & 0xffffu), (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
                        [&](int32_t tile, int32_t lo, int32_t len) {
-                const uint2 v = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
-                const unsigned int pi = atomicAdd(&L.npieces, 1u);
-                int sl = -1;
-                if (pi < BIN_PIECES) {
-                    sl = bin_slot(&L, (uint32_t)tile);
-                    if (sl >= 0) {
-                        const uint32_t rank = atomicAdd(&L.cnt[sl], 1u);
-                        L.val[pi] = v;
-                        L.where[pi] = ((uint32_t)sl << 16) | rank;
-                    } else {
-                        L.where[pi] = BIN_EMPTY;
-                    }
-                }
-                if (sl < 0) {       /* direct path */
-                    const uint32_t slot = atomicAdd(&B.tile_fill[tile], 1u);
-                    B.segs[B.tile_off[tile] + slot] = v;
-                }
+                if (j < 2)
+                    B.segs[B.tile_off[tile] + (j ? ps.y : ps.x)] = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
+                ++j;
             });
         });
-        __syncthreads();
-        /* reserve each (chunk, tile) range; cnt[] then holds its first slot */
-        for (int k = threadIdx.x; k < BIN_HASH; k += blockDim.x)
-            if (L.key[k] != BIN_EMPTY) L.cnt[k] = atomicAdd(&B.tile_fill[L.key[k]], L.cnt[k]);
-        __syncthreads();
-        const unsigned int np = L.npieces < BIN_PIECES ? L.npieces : BIN_PIECES;
-        for (unsigned int p = threadIdx.x; p < np; p += blockDim.x) {
-            const uint32_t w = L.where[p];
-            if (w == BIN_EMPTY) continue;
-            const uint32_t sl = w >> 16, rank = w & 0xffffu;
-            const uint32_t tile = L.key[sl];
-            B.segs[B.tile_off[tile] + L.cnt[sl] + rank] = L.val[p];
-        }
-        __syncthreads();
-        bin_reset(&L);
-        __syncthreads();
+    }
+    const unsigned long long nov = *B.n_ovf < (unsigned long long)B.ovf_cap ? *B.n_ovf : (unsigned long long)B.ovf_cap;
+    for (unsigned long long o = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; o < nov;
+         o += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint4 v = B.ovf[o];
+        B.segs[B.tile_off[v.x] + v.y] = make_uint2(v.z, v.w);
     }
     block_count(&B.counters[3], n_gaps, lds16);
 }
@@ -793,6 +823,7 @@ void p2m_launch_ring_counts(const BinArgs &b, hipStream_t s)
 static unsigned chunk_grid(unsigned long long n_items)
 {
     unsigned long long g = (n_items + BIN_CH - 1) / BIN_CH;
+    if (g >= 8) g &= ~7ULL;      /* a multiple of 8 for the XCD-aware split */
     return (unsigned)(g > 2048 ? 2048 : (g < 1 ? 1 : g));
 }
 void p2m_launch_seg_count(const BinArgs &b, hipStream_t s)

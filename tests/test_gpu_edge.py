@@ -66,6 +66,41 @@ def test_poles_and_large_discs(table, prec):
     assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= tol
 
 
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_COMP])
+def test_runs_over_many_tiles(table, prec):
+    """Discs ~0.35 rad across on 64-pixel tiles at nside 2048: every ring run
+    spans about 14 tiles, so most pieces lie beyond their item's second and go
+    through the binning's overflow list, whose first size (65536 entries) is
+    too small here: the binning grows it and counts again."""
+    print(f"\n[gpu edge] many-tile runs precision={prec}", flush=True)
+    h = np.array([
+        [0.0, 250.0, 0.0, 0, 0, 0, 200.0],
+        [0.0, -250.0, 0.0, 0, 0, 0, 200.0],
+        [0.0, 0.0, 250.0, 0, 0, 0, 200.0],
+        [60.0, 0.0, -240.0, 0, 0, 0, 200.0],
+        [0.0, 180.0, 180.0, 0, 0, 0, 120.0],
+        [40.0, -180.0, 170.0, 0, 0, 0, 200.0],
+        [-50.0, 170.0, -180.0, 0, 0, 0, 200.0],
+        [-30.0, -170.0, -175.0, 0, 0, 0, 200.0],
+    ], np.float32)
+    o = p2mo.Oracle(*table)
+    ref = o.paint(h, nside=2048, npix=64, nthreads=8)
+    with abi.Painter(nside=2048, npix_fs=64, precision=prec, membership=True) as p:
+        p.set_tile(64, 64)
+        p.set_run()
+        p.load_table(0, *table)
+        p.paint(h)
+        hp = p.maps()[0]
+        hc, hh, _, _ = p.membership()
+        st = p.stats()
+    assert st["n_segments"] > 140_000
+    assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+    nz = ref["hp"] != 0
+    assert nz.sum() > 50_000 and hc.sum() > 5_000_000
+    tol = 1e-6 if prec == abi.PREC_FP32_COMP else 1e-12
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= tol
+
+
 def test_invalid_radius_is_counted_and_skipped(table):
     # rth so large that rh > chi: asin(rh/chi) is undefined (quirk Q16)
     h = np.array([[0, 0, 220.0, 0, 0, 0, 5000.0],
