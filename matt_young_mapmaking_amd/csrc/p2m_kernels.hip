@@ -17,6 +17,7 @@
  *                    then one FP64 tile-to-map reduce per pixel
  */
 #include "p2m_kernels.h"
+#include "p2m_pieces.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -491,25 +492,6 @@ void p2m_launch_fs_paint(const PaintArgs &a, hipStream_t s)
 /* binning of (halo, ring) runs into ring tiles                           */
 /* ===================================================================== */
 
-/* tile pieces of one ring run [lo, lo+np) (wrapping at npr): f(tile, lo_in_tile, len) */
-template <typename F>
-__device__ static inline void run_pieces(int32_t lo, int32_t np, int32_t npr, int32_t tb, int log2w, F f)
-{
-    const int32_t a0 = lo, b0 = lo + np;
-    for (int piece = 0; piece < 2; ++piece) {
-        int32_t a, b;
-        if (piece == 0) { a = a0; b = b0 < npr ? b0 : npr; }
-        else { if (b0 <= npr) break; a = 0; b = b0 - npr; }
-        const int32_t c0 = a >> log2w, c1 = (b - 1) >> log2w;
-        for (int32_t col = c0; col <= c1; ++col) {
-            const int32_t s0 = col << log2w;
-            const int32_t pa = a > s0 ? a : s0;
-            const int32_t pb = b < s0 + (1 << log2w) ? b : s0 + (1 << log2w);
-            f(tb + col, pa - s0, pb - pa);
-        }
-    }
-}
-
 __global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= B.n_halo; i += (int64_t)gridDim.x * blockDim.x) {
@@ -710,42 +692,88 @@ __global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
     block_count(&B.counters[4], n_seg, lds16);
 }
 
-/* Scatter of every piece to its slot: segs[tile_off[tile] + slot]. */
+/* Scatter of every piece to its slot: segs[tile_off[tile] + slot].  A wave
+ * takes 128 consecutive items, two per lane, with the loads of both
+ * issued before either is used (the pass is bound by load latency). */
+#define EMIT_U 2
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
     __shared__ unsigned long long lds16[16];
     unsigned long long n_gaps = 0;
-    const unsigned long long ngroups = (B.n_items + 63) / 64;
-    unsigned long long gbeg, gend, gstep;
-    xcd_range(ngroups, gbeg, gend, gstep, blockDim.x >> 6, threadIdx.x >> 6);
-    for (unsigned long long g = gbeg; g < gend; g += gstep) {
-        const unsigned long long c = g * 64;
-        const unsigned long long e = B.n_items - c < 64 ? B.n_items : c + 64;
-        wave_range(B.ring_off, B.group_halo, B.n_halo, c, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
-            const uint2 run = B.runs[item];
-            if (run.y == 0) {
-                /* an empty ring between non-empty rings of one disc, counted
-                 * once per gap (the row misalignment of discedge2fulldisc,
-                 * quirk Q15) */
-                if (k > 0 && B.runs[item - 1].y != 0) {
-                    const unsigned long long endi = B.ring_off[hi + 1];
-                    for (unsigned long long j = item + 1; j < endi; ++j)
-                        if (B.runs[j].y != 0) { n_gaps++; break; }
-                }
-                return;
-            }
-            const uint2 ps = B.pos[item];
-            const int32_t ring = (int32_t)(run.x >> 16);
-            int32_t npr, kshift; int64_t first;
-            ring_info(B.nside, ring, npr, kshift, first);
+    const int lane = __lane_id();
+    const unsigned long long nunit = (B.n_items + 64 * EMIT_U - 1) / (64 * EMIT_U);
+    unsigned long long ubeg, uend, ustep;
+    xcd_range(nunit, ubeg, uend, ustep, blockDim.x >> 6, threadIdx.x >> 6);
+    for (unsigned long long un = ubeg; un < uend; un += ustep) {
+        const unsigned long long base = un * (64 * EMIT_U);
+        unsigned long long item[EMIT_U];
+        uint2 run[EMIT_U], ps[EMIT_U];
+        int64_t hgrp[EMIT_U];
+        unsigned long long off[EMIT_U];
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
+            item[u] = base + 64 * u + lane;
+            const unsigned long long it = item[u] < B.n_items ? item[u] : B.n_items - 1;
+            run[u] = B.runs[it];
+            ps[u] = B.pos[it];
+            unsigned long long c = base + 64 * u;
+            if (c >= B.n_items) c = (B.n_items - 1) & ~63ULL;
+            hgrp[u] = B.group_halo[c >> 6];
+        }
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
+            const int64_t hl = hgrp[u] + lane;
+            off[u] = hl <= B.n_halo ? B.ring_off[hl] : ~0ULL;
+        }
+        int64_t hi[EMIT_U];
+        unsigned long long kk[EMIT_U];
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
             int j = 0;
-            run_pieces((int32_t)(run.x & 0xffffu), (int32_t)run.y, npr, B.tile_base[ring - 1], B.log2w,
-                       [&](int32_t tile, int32_t lo, int32_t len) {
-                if (j < 2)
-                    B.segs[B.tile_off[tile] + (j ? ps.y : ps.x)] = make_uint2((uint32_t)hi, ((uint32_t)lo << 16) | (uint32_t)len);
-                ++j;
-            });
-        });
+            for (int step = 32; step > 0; step >>= 1) {
+                const unsigned long long o = __shfl(off[u], j + step);
+                if (o <= item[u]) j += step;
+            }
+            hi[u] = hgrp[u] + j;
+            kk[u] = item[u] - __shfl(off[u], j);
+        }
+        int32_t tb[EMIT_U], npr[EMIT_U];
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
+            const int32_t ring = (int32_t)(run[u].x >> 16);
+            int32_t kshift; int64_t first;
+            const int32_t rr = ring >= 1 ? ring : 1;
+            ring_info(B.nside, rr, npr[u], kshift, first);
+            tb[u] = B.tile_base[rr - 1];
+        }
+        int32_t tile[EMIT_U][2], plo[EMIT_U][2], len[EMIT_U][2];
+        int npc[EMIT_U];
+        unsigned long long to[EMIT_U][2];
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
+            const bool live = item[u] < B.n_items && run[u].y != 0;
+            npc[u] = live ? first_two_pieces((int32_t)(run[u].x & 0xffffu), (int32_t)run[u].y, npr[u], tb[u], B.log2w,
+                                             tile[u], plo[u], len[u]) : 0;
+            to[u][0] = npc[u] > 0 ? B.tile_off[tile[u][0]] : 0;
+            to[u][1] = npc[u] > 1 ? B.tile_off[tile[u][1]] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
+            if (npc[u] > 0)
+                B.segs[to[u][0] + ps[u].x] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][0] << 16) | (uint32_t)len[u][0]);
+            if (npc[u] > 1)
+                B.segs[to[u][1] + ps[u].y] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][1] << 16) | (uint32_t)len[u][1]);
+        }
+        /* an empty ring between non-empty rings of one disc, counted once per
+         * gap (the row misalignment of discedge2fulldisc, quirk Q15) */
+#pragma unroll
+        for (int u = 0; u < EMIT_U; ++u) {
+            if (item[u] < B.n_items && run[u].y == 0 && kk[u] > 0 && B.runs[item[u] - 1].y != 0) {
+                const unsigned long long endi = B.ring_off[hi[u] + 1];
+                for (unsigned long long j = item[u] + 1; j < endi; ++j)
+                    if (B.runs[j].y != 0) { n_gaps++; break; }
+            }
+        }
     }
     const unsigned long long nov = *B.n_ovf < (unsigned long long)B.ovf_cap ? *B.n_ovf : (unsigned long long)B.ovf_cap;
     for (unsigned long long o = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; o < nov;

@@ -1,0 +1,27 @@
+"""The enumeration of a ring run's tile pieces (p2m_pieces.h): the binning
+assigns segment slots with run_pieces and writes the first two pieces of
+every run with first_two_pieces, so the two must agree piece for piece.
+Checked on the host over every run start and length (strided on long
+rings), ring lengths that are and are not multiples of the tile width, and
+runs that wrap past the end of the ring."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "matt_young_mapmaking_amd", "csrc")
+
+
+def test_first_two_pieces_match_run_pieces(tmp_path):
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not found")
+    exe = tmp_path / "pieces_check"
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "pieces_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    assert "0 mismatches" in r.stdout
