@@ -1075,17 +1075,22 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
         }
         lds_barrier();
         const int64_t pix0 = G.first + G.ip0;
-        for (int q = tid; q < G.width; q += blockDim.x) {
-            for (int m = 0; m < NM; ++m) {
-                const double v = acc[m * W + q];
-                double *dst = A.map64[m] + pix0 + q;
-                *dst = A.accumulate ? *dst + v : v;
+        /* branch on accumulate outside the loops (no speculative map loads) */
+        if (A.accumulate) {
+            for (int q = tid; q < G.width; q += blockDim.x) {
+                for (int m = 0; m < NM; ++m) A.map64[m][pix0 + q] += acc[m * W + q];
+                if (A.count) {
+                    A.count[pix0 + q] += lcount[q];
+                    A.hash[pix0 + q] += lhash[q];
+                }
             }
-            if (A.count) {
-                uint32_t *dc = A.count + pix0 + q;
-                unsigned long long *dh = A.hash + pix0 + q;
-                *dc = A.accumulate ? *dc + lcount[q] : lcount[q];
-                *dh = A.accumulate ? *dh + lhash[q] : lhash[q];
+        } else {
+            for (int q = tid; q < G.width; q += blockDim.x) {
+                for (int m = 0; m < NM; ++m) A.map64[m][pix0 + q] = acc[m * W + q];
+                if (A.count) {
+                    A.count[pix0 + q] = lcount[q];
+                    A.hash[pix0 + q] = lhash[q];
+                }
             }
         }
         lds_barrier();
@@ -1287,17 +1292,27 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
         }
         lds_barrier();
         const int64_t pix0 = G.first + G.ip0;
-        for (int q = tid; q < G.width; q += blockDim.x) {
-            for (int m = 0; m < NM; ++m) {
-                const double v = acc[m * W + q];
-                float *dst = A.map32[m] + pix0 + q;
-                *dst = A.accumulate ? (float)((double)*dst + v) : (float)v;
+        /* the branch on accumulate is taken outside the loops: a select
+         * inside lets the compiler load the map speculatively (a map read and
+         * a wait on it per tile) even when the tile only stores */
+        if (A.accumulate) {
+            for (int q = tid; q < G.width; q += blockDim.x) {
+                for (int m = 0; m < NM; ++m) {
+                    float *dst = A.map32[m] + pix0 + q;
+                    *dst = (float)((double)*dst + acc[m * W + q]);
+                }
+                if (A.count) {
+                    A.count[pix0 + q] += lcount[q];
+                    A.hash[pix0 + q] += lhash[q];
+                }
             }
-            if (A.count) {
-                uint32_t *dc = A.count + pix0 + q;
-                unsigned long long *dh = A.hash + pix0 + q;
-                *dc = A.accumulate ? *dc + lcount[q] : lcount[q];
-                *dh = A.accumulate ? *dh + lhash[q] : lhash[q];
+        } else {
+            for (int q = tid; q < G.width; q += blockDim.x) {
+                for (int m = 0; m < NM; ++m) A.map32[m][pix0 + q] = (float)acc[m * W + q];
+                if (A.count) {
+                    A.count[pix0 + q] = lcount[q];
+                    A.hash[pix0 + q] = lhash[q];
+                }
             }
         }
         /* flush the deferred-pixel buffer when half full */
