@@ -129,7 +129,7 @@ int p2m_fetch_membership(p2m_ctx *ctx, uint32_t *hp_count, uint64_t *hp_hash,
 int p2m_get_stats(p2m_ctx *ctx, p2m_stats *st);
 
 /* Tile shape of the tiled painter: width in pixels along a ring (power of 2,
- * 64..4096) and workgroup size (a multiple of 64, 64..512; 64..256 for
+ * 64..2048) and workgroup size (a multiple of 64, 64..512; 64..256 for
  * P2M_PREC_FP32_COMP).
  * 0,0 = autotune on first paint. */
 int p2m_set_tile(p2m_ctx *ctx, int32_t tile_w, int32_t tile_wg);

@@ -459,7 +459,7 @@ int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
 {
     if (!c) return P2M_E_INVALID;
     if (tile_w == 0 && tile_wg == 0) { c->tuned = false; return 0; }
-    if (!is_pow2(tile_w) || tile_w < 64 || tile_w > 4096) return P2M_E_INVALID;
+    if (!is_pow2(tile_w) || tile_w < 64 || tile_w > (1 << p2m_tile_max_log2w())) return P2M_E_INVALID;
     if (tile_wg < 64 || tile_wg > 512 || tile_wg % 64) return P2M_E_INVALID;
     if (c->cfg.precision == P2M_PREC_FP32_COMP && tile_wg > 256) return P2M_E_INVALID;
     int l = 0;
@@ -970,6 +970,7 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long lo
     }
     float best_ms = 1e30f;
     for (auto &cd : cand) {
+        if (cd[0] > p2m_tile_max_log2w()) continue;
         if (p2m_tile_lds_bytes(mode_ld, c->cfg.n_maps, cd[0], c->d_count != nullptr, c->g.nrt) > 160 * 1024) continue;
         int rc = build_tiles(c, cd[0]); if (rc) return rc;
         c->wg = cd[1]; c->tuned = true;

@@ -156,6 +156,7 @@ int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin,
                    int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
                    hipStream_t s);
 int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
+int p2m_tile_max_log2w(void);
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int nrt);
 int p2m_tile_occupancy(int mode, int n_maps, int wg, size_t lds);
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);

@@ -893,8 +893,8 @@ __device__ static inline void lds_barrier()
 }
 
 #define FSEG 256                   /* segments staged per batch */
-#define FB_LDS 128                 /* deferred pixels buffered in LDS */
-#define PMAX 4096                  /* pixels per batch (>= the widest tile) */
+#define FB_LDS 32                  /* deferred pixels buffered in LDS (flushed half full) */
+#define PMAX 3072                  /* pixels per batch (>= the widest tile, 2048) */
 
 /* the part of a batch both painters share: segment starts and the pixel
  * -> segment map */
@@ -1119,7 +1119,7 @@ struct FastSeg {
     double dphi0[FSEG];            /* phi offset of the ring's pixel 0 from the halo centre */
     uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
-    uint32_t lo[FSEG];
+    uint16_t lo[FSEG];             /* first in-tile pixel (< 2048) */
     float A[FSEG], B[FSEG], thr_hi[FSEG], chi2[FSEG];
     float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];   /* corner weights times the velocity factor */
     int32_t colbase[FSEG];
@@ -1161,7 +1161,8 @@ __device__ static inline float log1p_small_rcp(float q)
 }
 
 template <int NM>
-__global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
+__global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *__restrict__ tdesc,
+                                                   const unsigned long long *__restrict__ toff)
 {
     extern __shared__ unsigned char smem_raw[];
     const int W = 1 << A.log2w;
@@ -1192,9 +1193,16 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
 
     int64_t tbeg, tend, tstep;
     tile_range(A, tbeg, tend, tstep);
+    /* the descriptor and segment range of the next tile are requested (as
+     * scalar loads: read-only, uniform) once the current tile's first
+     * segments are, so they arrive during its staging */
+    TileDesc Gn;
+    uint64_t sbn = 0, sen = 0;
+    if (tbeg < tend) { Gn = tdesc[tbeg]; sbn = toff[tbeg]; sen = toff[tbeg + 1]; }
     for (int64_t t = tbeg; t < tend; t += tstep) {
-        const TileDesc G = A.desc[t];
-        const uint64_t sbeg = A.tile_off[t], send = A.tile_off[t + 1];
+        const TileDesc G = Gn;
+        const uint64_t sbeg = sbn, send = sen;
+        bool pending = t + tstep < tend;
         const double kphi = G.kphi;
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
@@ -1205,8 +1213,12 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
             int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
             if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
+            const uint2 sg = tid < nstage ? A.segs[b0 + tid] : make_uint2(0u, 0u);
+            if (pending) {
+                pending = false;
+                Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
+            }
             if (tid < nstage) {
-                const uint2 sg = A.segs[b0 + tid];
                 const TileHalo &h = A.hot[sg.x];
                 sl->halo[tid] = sg.x;
                 sl->lo[tid] = sg.y >> 16;
@@ -1289,6 +1301,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A)
                     atomicAdd(&lhash[q], hh[s]);
                 }
             }
+        }
+        if (pending) {     /* a tile without segments */
+            Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
         }
         lds_barrier();
         const int64_t pix0 = G.first + G.ip0;
@@ -1390,6 +1405,15 @@ int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+/* widest tile the painters take: a segment (at most one tile wide) must fit
+ * in one batch of PMAX pixels */
+int p2m_tile_max_log2w(void)
+{
+    int l = 6;
+    while ((2 << l) <= PMAX) ++l;
+    return l;
+}
+
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int nrt)
 {
     size_t W = (size_t)1 << log2w;
@@ -1425,7 +1449,7 @@ int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int
     size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr, a.g.nrt);
     if (lds > 160 * 1024 || wg > 512 || (mode == 2 && wg > 256)) return -1;
 #define P2M_TE(NMv) case NMv: hipLaunchKernelGGL((k_tile_exact<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
-#define P2M_TF(NMv) case NMv: hipLaunchKernelGGL((k_tile_fast<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
+#define P2M_TF(NMv) case NMv: hipLaunchKernelGGL((k_tile_fast<NMv>), dim3(grid), dim3(wg), lds, s, a, a.desc, a.tile_off); break;
     if (mode == 1) {
         switch (n_maps) { P2M_TE(1) P2M_TE(2) P2M_TE(3) P2M_TE(4) P2M_TE(5) P2M_TE(6) default: return -1; }
     } else {

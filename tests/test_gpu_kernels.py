@@ -125,11 +125,13 @@ def test_multimap_equals_single_maps(table):
 def test_chunking_tiles_and_repeatability(table):
     h = catalog.config_catalog("C3", n=300000)
     ref = None
-    for chunk, tile in ((0, None), (1 << 16, (256, 256)), (70001, (2048, 256)), (0, (512, 128)), (0, (4096, 64))):
+    for chunk, tile in ((0, None), (1 << 16, (256, 256)), (70001, (2048, 256)), (0, (512, 128)), (0, (2048, 64))):
         with abi.Painter(nside=4096, npix_fs=0, precision=abi.PREC_FP32_COMP, membership=True,
                          chunk_halos=chunk) as p:
             if tile:
                 p.set_tile(*tile)
+            with pytest.raises(Exception):
+                p.set_tile(4096, 64)         # wider than a batch holds
             p.set_run()
             p.load_table(0, *table)
             p.paint(h)
