@@ -1126,28 +1126,19 @@ struct FastSeg {
     uint32_t nfb;
 };
 
-/* sin^2(x), |x| <= pi/2, FP32 Taylor polynomials (to x^11 / x^10) */
+/* sin^2(x) for |x| <= pi/2: the odd Taylor polynomial of sin to x^15 (the
+ * next term is below 6e-12 there), squared; one branch-free sequence */
 __device__ static inline float sin2f(float x)
 {
-    float ax = __builtin_fabsf(x);
-    if (ax <= 0.78539816f) {
-        float z = x * x;
-        float p = __builtin_fmaf(z, -2.5052108e-8f, 2.7557319e-6f);
-        p = __builtin_fmaf(z, p, -1.9841270e-4f);
-        p = __builtin_fmaf(z, p, 8.3333333e-3f);
-        p = __builtin_fmaf(z, p, -1.6666667e-1f);
-        float sn = __builtin_fmaf(x * z, p, x);
-        return sn * sn;
-    }
-    float y = 1.5707963f - ax;
-    float z = y * y;
-    float p = __builtin_fmaf(z, 2.0876757e-9f, -2.7557319e-7f);
-    p = __builtin_fmaf(z, p, 2.4801587e-5f);
-    p = __builtin_fmaf(z, p, -1.3888889e-3f);
-    p = __builtin_fmaf(z, p, 4.1666667e-2f);
-    p = __builtin_fmaf(z, p, -0.5f);
-    float cs = __builtin_fmaf(z, p, 1.0f);
-    return cs * cs;
+    const float z = x * x;
+    float p = __builtin_fmaf(z, -7.6471637e-13f, 1.6059044e-10f);
+    p = __builtin_fmaf(z, p, -2.5052108e-8f);
+    p = __builtin_fmaf(z, p, 2.7557319e-6f);
+    p = __builtin_fmaf(z, p, -1.9841270e-4f);
+    p = __builtin_fmaf(z, p, 8.3333333e-3f);
+    p = __builtin_fmaf(z, p, -1.6666667e-1f);
+    const float sn = __builtin_fmaf(x * z, p, x);
+    return sn * sn;
 }
 
 /* ln(1+q) for 0 <= q < 0.08 (2 atanh(q/(2+q))) with the hardware reciprocal */
