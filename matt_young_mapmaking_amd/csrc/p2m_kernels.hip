@@ -623,7 +623,9 @@ __device__ static inline void xcd_range(unsigned long long n, unsigned long long
     }
 }
 
-__global__ void __launch_bounds__(256) k_seg_count(BinArgs B)
+/* eight waves per SIMD (64 VGPRs, a few spilled): the pass waits on loads
+ * and LDS atomics far more than it computes, so occupancy pays */
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_seg_count(BinArgs B)
 {
     __shared__ CountLDS L;
     __shared__ unsigned long long lds16[16];
