@@ -86,7 +86,7 @@ struct p2m_ctx {
     uint32_t *d_flat_idx = nullptr;
     uint32_t *d_nr = nullptr;
     unsigned long long *d_ring_off = nullptr;
-    uint2 *d_runs = nullptr, *d_pos = nullptr;   /* per (halo, ring) item: run, piece slots */
+    uint32_t *d_runs = nullptr, *d_pos = nullptr;   /* per (halo, ring) item: run, first piece's slot */
     int64_t runs_cap = 0;
     uint4 *d_ovf = nullptr;                       /* pieces beyond an item's second */
     int64_t ovf_cap = 0;
@@ -635,12 +635,14 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         if ((int64_t)n_items > c->runs_cap || !c->d_runs || !c->d_pos) {
             dfree(c->d_runs); dfree(c->d_pos); c->d_runs = nullptr; c->d_pos = nullptr; c->runs_cap = 0;
             int64_t cap = (int64_t)(n_items * 1.25) + 1024;
-            if (dmalloc((void **)&c->d_runs, sizeof(uint2) * cap)) return P2M_E_NOMEM;
-            if (dmalloc((void **)&c->d_pos, sizeof(uint2) * cap)) return P2M_E_NOMEM;
+            if (dmalloc((void **)&c->d_runs, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
+            if (dmalloc((void **)&c->d_pos, sizeof(uint32_t) * cap)) return P2M_E_NOMEM;
             c->runs_cap = cap;
         }
-        if (!c->d_ovf) {
-            const int64_t cap = 1 << 16;
+        if (!c->d_ovf || c->ovf_cap < (int64_t)(n_items / 8) + (1 << 16)) {
+            /* about 8% of the items cross a tile boundary at the default shape */
+            dfree(c->d_ovf); c->d_ovf = nullptr; c->ovf_cap = 0;
+            const int64_t cap = (int64_t)(n_items / 8) + (1 << 16);
             if (dmalloc((void **)&c->d_ovf, sizeof(uint4) * cap)) return P2M_E_NOMEM;
             c->ovf_cap = cap;
         }

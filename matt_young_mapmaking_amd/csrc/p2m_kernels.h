@@ -70,13 +70,13 @@ struct BinArgs {
     unsigned long long n_items;   /* total (halo, ring) items */
     uint32_t *group_halo;         /* halo of item 64 g (n_items / 64 + 1 entries) */
     const double *ring_z;         /* ring2z of every ring */
-    uint2 *runs;                  /* per item: (first in-ring pixel | ring << 16, pixel count) */
-    uint2 *pos;                   /* per item: slots of its first two tile pieces within their tiles */
+    uint32_t *runs;               /* per item: first in-ring pixel | pixel count << 16 */
+    uint32_t *pos;                /* per item: slot of its first tile piece within its tile */
     const int32_t *tile_base;     /* first tile of ring r+1 */
     uint32_t *tile_cnt;           /* segments per tile */
     const unsigned long long *tile_off;  /* exclusive scan of tile_cnt */
     uint2 *segs;                  /* (halo, lo<<16 | len) */
-    uint4 *ovf;                   /* pieces beyond an item's second: (tile, slot, halo, lo<<16 | len) */
+    uint4 *ovf;                   /* pieces after an item's first: (tile, slot, halo, lo<<16 | len) */
     unsigned long long *n_ovf;    /* entries of ovf (one counter, not sharded) */
     int64_t ovf_cap;
     unsigned long long *counters; /* [0] listed, [3] ring gaps, [4] segments */

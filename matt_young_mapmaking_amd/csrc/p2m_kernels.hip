@@ -523,7 +523,7 @@ struct BinHash {
 };
 
 struct CountLDS : BinHash {
-    uint2 prec[BIN_CH];          /* per item of the chunk: its two pieces' slot codes */
+    uint32_t prec[BIN_CH];       /* per item of the chunk: its first piece's slot code */
 };
 
 __device__ static inline int bin_slot(BinHash *L, uint32_t tile)
@@ -590,11 +590,12 @@ __global__ void __launch_bounds__(256) k_group_heads(BinArgs B)
 }
 
 /* Per (halo, ring) item: the query_disc pixel run, its tile pieces and the
- * slot of each piece within its tile's segment list.  A piece's slot code is
- * (hash slot << 16 | rank within the chunk's pieces of that tile) until the
- * chunk reserves its range in the tile, BIN_DIRECT | slot when reserved on
- * its own, BIN_EMPTY for none.  Pieces beyond an item's second (runs
- * spanning three or more tiles) are listed in ovf with their slot. */
+ * slot of each piece within its tile's segment list.  The first piece's slot
+ * code is (hash slot << 16 | rank within the chunk's pieces of that tile)
+ * until the chunk reserves its range in the tile, BIN_DIRECT | slot when
+ * reserved on its own, BIN_EMPTY for none.  Later pieces (runs crossing a
+ * tile boundary, about 8% of the items) take their slot on their own and are
+ * listed in ovf with it. */
 __device__ static inline uint32_t slot_of(const BinHash &L, uint32_t code)
 {
     if (code == BIN_EMPTY) return 0u;
@@ -650,20 +651,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                 ring_info(B.nside, ring, npr, kshift, first);
                 np = disc_ring_run_z(h, B.ring_z[ring - 1], npr, kshift, lo);
             }
-            /* (first pixel | ring << 16, count): both below 2^16 for nside <= 8192 */
-            B.runs[item] = make_uint2((uint32_t)lo | ((uint32_t)ring << 16), (uint32_t)np);
-            uint32_t code[2] = {BIN_EMPTY, BIN_EMPTY};
+            /* first pixel | count << 16: lo < npr and np <= npr <= 32768 for nside <= 8192 */
+            B.runs[item] = (uint32_t)lo | ((uint32_t)np << 16);
+            uint32_t code = BIN_EMPTY;
             if (np > 0) {
                 n_list += (unsigned long long)np;
                 int j = 0;
                 run_pieces(lo, np, npr, B.tile_base[ring - 1], B.log2w, [&](int32_t tile, int32_t plo, int32_t len) {
-                    const int sl = j < 2 ? bin_slot(&L, (uint32_t)tile) : -1;
+                    const int sl = j == 0 ? bin_slot(&L, (uint32_t)tile) : -1;
                     if (sl >= 0) {
-                        code[j] = ((uint32_t)sl << 16) | atomicAdd(&L.cnt[sl], 1u);
+                        code = ((uint32_t)sl << 16) | atomicAdd(&L.cnt[sl], 1u);
                     } else {
                         const uint32_t slot = atomicAdd(&B.tile_cnt[tile], 1u);
-                        if (j < 2) {
-                            code[j] = BIN_DIRECT | slot;
+                        if (j == 0) {
+                            code = BIN_DIRECT | slot;
                         } else {
                             const unsigned long long o = atomicAdd(B.n_ovf, 1ULL);
                             if (o < (unsigned long long)B.ovf_cap)
@@ -674,7 +675,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                     n_seg++;
                 });
             }
-            L.prec[item - c0] = make_uint2(code[0], code[1]);
+            L.prec[item - c0] = code;
         });
         __syncthreads();
         /* reserve each (chunk, tile) range; cnt[] then holds its first slot */
@@ -682,10 +683,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
             if (L.key[k] != BIN_EMPTY) L.cnt[k] = atomicAdd(&B.tile_cnt[L.key[k]], L.cnt[k]);
         __syncthreads();
         const unsigned long long nin = B.n_items - c0 < BIN_CH ? B.n_items - c0 : BIN_CH;
-        for (unsigned long long k = threadIdx.x; k < nin; k += blockDim.x) {
-            const uint2 cd = L.prec[k];
-            B.pos[c0 + k] = make_uint2(slot_of(L, cd.x), slot_of(L, cd.y));
-        }
+        for (unsigned long long k = threadIdx.x; k < nin; k += blockDim.x) B.pos[c0 + k] = slot_of(L, L.prec[k]);
         __syncthreads();
         bin_reset(&L);
         __syncthreads();
@@ -695,8 +693,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
 }
 
 /* Scatter of every piece to its slot: segs[tile_off[tile] + slot].  A wave
- * takes 128 consecutive items, two per lane, with the loads of both
- * issued before either is used (the pass is bound by load latency). */
+ * takes 128 consecutive items, two per lane, with the loads of both issued
+ * before either is used (the pass is bound by load latency); the pieces
+ * after an item's first come from ovf. */
 #define EMIT_U 2
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
@@ -709,7 +708,7 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
     for (unsigned long long un = ubeg; un < uend; un += ustep) {
         const unsigned long long base = un * (64 * EMIT_U);
         unsigned long long item[EMIT_U];
-        uint2 run[EMIT_U], ps[EMIT_U];
+        uint32_t run[EMIT_U], ps[EMIT_U];
         int64_t hgrp[EMIT_U];
         unsigned long long off[EMIT_U];
 #pragma unroll
@@ -742,38 +741,35 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
         int32_t tb[EMIT_U], npr[EMIT_U];
 #pragma unroll
         for (int u = 0; u < EMIT_U; ++u) {
-            const int32_t ring = (int32_t)(run[u].x >> 16);
+            const int64_t h = hi[u] < B.n_halo ? hi[u] : B.n_halo - 1;
+            int32_t ring = B.halos[h].irmin + (int32_t)kk[u];
+            ring = ring < 1 ? 1 : (ring > 4 * B.nside - 1 ? 4 * B.nside - 1 : ring);
             int32_t kshift; int64_t first;
-            const int32_t rr = ring >= 1 ? ring : 1;
-            ring_info(B.nside, rr, npr[u], kshift, first);
-            tb[u] = B.tile_base[rr - 1];
+            ring_info(B.nside, ring, npr[u], kshift, first);
+            tb[u] = B.tile_base[ring - 1];
         }
         int32_t tile[EMIT_U][2], plo[EMIT_U][2], len[EMIT_U][2];
-        int npc[EMIT_U];
-        unsigned long long to[EMIT_U][2];
+        bool live[EMIT_U];
+        unsigned long long to[EMIT_U];
 #pragma unroll
         for (int u = 0; u < EMIT_U; ++u) {
-            const bool live = item[u] < B.n_items && run[u].y != 0;
-            npc[u] = live ? first_two_pieces((int32_t)(run[u].x & 0xffffu), (int32_t)run[u].y, npr[u], tb[u], B.log2w,
-                                             tile[u], plo[u], len[u]) : 0;
-            to[u][0] = npc[u] > 0 ? B.tile_off[tile[u][0]] : 0;
-            to[u][1] = npc[u] > 1 ? B.tile_off[tile[u][1]] : 0;
+            const int32_t np = (int32_t)(run[u] >> 16);
+            live[u] = item[u] < B.n_items && np != 0;
+            if (live[u]) first_two_pieces((int32_t)(run[u] & 0xffffu), np, npr[u], tb[u], B.log2w, tile[u], plo[u], len[u]);
+            to[u] = live[u] ? B.tile_off[tile[u][0]] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < EMIT_U; ++u) {
-            if (npc[u] > 0)
-                B.segs[to[u][0] + ps[u].x] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][0] << 16) | (uint32_t)len[u][0]);
-            if (npc[u] > 1)
-                B.segs[to[u][1] + ps[u].y] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][1] << 16) | (uint32_t)len[u][1]);
-        }
+        for (int u = 0; u < EMIT_U; ++u)
+            if (live[u])
+                B.segs[to[u] + ps[u]] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][0] << 16) | (uint32_t)len[u][0]);
         /* an empty ring between non-empty rings of one disc, counted once per
          * gap (the row misalignment of discedge2fulldisc, quirk Q15) */
 #pragma unroll
         for (int u = 0; u < EMIT_U; ++u) {
-            if (item[u] < B.n_items && run[u].y == 0 && kk[u] > 0 && B.runs[item[u] - 1].y != 0) {
+            if (item[u] < B.n_items && (run[u] >> 16) == 0 && kk[u] > 0 && (B.runs[item[u] - 1] >> 16) != 0) {
                 const unsigned long long endi = B.ring_off[hi[u] + 1];
                 for (unsigned long long j = item[u] + 1; j < endi; ++j)
-                    if (B.runs[j].y != 0) { n_gaps++; break; }
+                    if ((B.runs[j] >> 16) != 0) { n_gaps++; break; }
             }
         }
     }

@@ -69,9 +69,9 @@ def test_poles_and_large_discs(table, prec):
 @pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_COMP])
 def test_runs_over_many_tiles(table, prec):
     """Discs ~0.35 rad across on 64-pixel tiles at nside 2048: every ring run
-    spans about 14 tiles, so most pieces lie beyond their item's second and go
-    through the binning's overflow list, whose first size (65536 entries) is
-    too small here: the binning grows it and counts again."""
+    spans about 14 tiles, so most pieces come after their item's first and go
+    through the binning's overflow list, whose first size (n_items / 8 +
+    65536 entries) is too small here: the binning grows it and counts again."""
     print(f"\n[gpu edge] many-tile runs precision={prec}", flush=True)
     h = np.array([
         [0.0, 250.0, 0.0, 0, 0, 0, 200.0],
