@@ -56,7 +56,7 @@ typedef struct {
     int32_t profile;      /* 1 tSZ (v = 1), 2 kSZ (v = -v_r/c) (pksc.f90:287-295) */
     int32_t n_maps;       /* tables painted in one catalogue pass (1..6) */
     int32_t precision;    /* P2M_PREC_* */
-    int64_t chunk_halos;  /* halos per streamed chunk (0: 1<<24; at most 1<<30) */
+    int64_t chunk_halos;  /* halos per streamed chunk (0: 1<<26; at most 1<<30) */
     int32_t device;       /* HIP device ordinal */
     int32_t flags;        /* P2M_FLAG_* */
 } p2m_config;

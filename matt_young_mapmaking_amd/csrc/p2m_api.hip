@@ -245,7 +245,7 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess) c->n_cu = ncu;
         else (void)hipGetLastError();
     }
-    if (c->cfg.chunk_halos <= 0) c->cfg.chunk_halos = 1 << 24;
+    if (c->cfg.chunk_halos <= 0) c->cfg.chunk_halos = 1 << 26;
     if (c->cfg.chunk_halos > (1LL << 30)) c->cfg.chunk_halos = 1LL << 30;   /* chunk indices are 32-bit */
     memset(&c->st, 0, sizeof(c->st));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return P2M_E_HIP; }

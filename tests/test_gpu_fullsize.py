@@ -21,8 +21,9 @@ pytestmark = pytest.mark.gpu
 TABLES_C5 = [(148.0, 0.0), (200.0, 0.0), (201.0, 0.0), (218.0, 0.0), (353.0, 0.1), (545.0, 0.0)]
 
 
-def paint(dev, n, nside, tabs, prec, membership, tile):
-    p = abi.Painter(nside=nside, npix_fs=1024, precision=prec, n_maps=len(tabs), membership=membership)
+def paint(dev, n, nside, tabs, prec, membership, tile, chunk=0):
+    p = abi.Painter(nside=nside, npix_fs=1024, precision=prec, n_maps=len(tabs), membership=membership,
+                    chunk_halos=chunk)
     p.set_tile(*tile)
     p.set_run()
     for k, t in enumerate(tabs):
@@ -53,9 +54,10 @@ def test_fullsize_fp32_against_fp64(wl):
     tabs = [abi.make_maptable(model=1, nu=nu, alpha=a) for nu, a in freqs]
     tile = (1024, 256) if len(tabs) == 1 else (512, 256)
     member = wl == "C4"
-    p32 = paint(dev, n, nside, tabs, abi.PREC_FP32_COMP, member, tile)
+    chunk = (1 << 24) if wl == "C4" else 0       # C4 is defined as 2^24-halo batches
+    p32 = paint(dev, n, nside, tabs, abi.PREC_FP32_COMP, member, tile, chunk)
     print(f"[gpu fullsize] {wl} fp32 painted", flush=True)
-    p64 = paint(dev, n, nside, tabs, abi.PREC_FP64, member, tile)
+    p64 = paint(dev, n, nside, tabs, abi.PREC_FP64, member, tile, chunk)
     print(f"[gpu fullsize] {wl} fp64 painted", flush=True)
     s32, s64 = p32.stats(), p64.stats()
     assert s32["n_upd_hp"] == s64["n_upd_hp"] and s32["n_painted"] == s64["n_painted"]
