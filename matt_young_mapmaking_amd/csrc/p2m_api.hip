@@ -721,6 +721,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
         t.band_c = 4e-6f;
         t.inv_node2f = c->d_inv_node2f;
+        t.n_prof = c->n_prof;
         t.fb = c->d_fb; t.fb_count = pc + 6; t.fb_cap = c->fb_cap;
         for (int m = 0; m < c->cfg.n_maps; ++m) {
             t.tab_flat[m] = c->d_flat[m]; t.tab_prof[m] = c->d_prof[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];

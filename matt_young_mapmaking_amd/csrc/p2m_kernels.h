@@ -104,6 +104,7 @@ struct TileArgs {
     TabGeom g;
     const float *tab_flat[P2M_MAXMAPS];
     const float *tab_prof[P2M_MAXMAPS];  /* the profile's slice of the table, zero-padded */
+    int64_t n_prof;               /* floats in each slice */
     int64_t n_flat;
     const double *inv_node2;      /* exp(-2 (lr0 + k drt)), k = 0..nrt */
     const float *inv_node2f;      /* the same in float */

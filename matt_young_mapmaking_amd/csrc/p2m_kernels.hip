@@ -1170,7 +1170,12 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
     const float lo_band = (float)((1.0 - (double)A.band_c) / (1.0 + (double)A.band_c));
 
     const int nrt = A.g.nrt;
-    const int64_t dmh = nrt, dch = (int64_t)nrt * A.g.nmht;
+    /* the table slices as raw buffers: a 32-bit byte offset per pixel, the
+     * corner strides as scalar offsets, reads past the slice return 0 */
+    __amdgpu_buffer_rsrc_t trs[NM];
+    for (int m = 0; m < NM; ++m)
+        trs[m] = __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_prof[m], (short)0, (int)(A.n_prof * 4), 0x00020000);
+    const int so_m = nrt * 4, so_c = nrt * A.g.nmht * 4, so_mc = so_m + so_c;
     const int tid = threadIdx.x;
     unsigned long long n_upd = 0, n_fb = 0;
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
@@ -1270,15 +1275,18 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 ir = ir < 1 ? 1 : (ir > nrt - 1 ? nrt - 1 : ir);
                 const float fr = clampr ? A.fr_clamp
                                         : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
-                const int64_t c0 = (int64_t)sl->colbase[s] + (ir - 1);
+                const int vo = (sl->colbase[s] + (ir - 1)) * 4;
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
                 n_upd++;
                 for (int m = 0; m < NM; ++m) {
-                    const float *T = A.tab_prof[m];
-                    const float a0 = T[c0], a1 = T[c0 + 1];
-                    const float b0v = T[c0 + dmh], b1 = T[c0 + dmh + 1];
-                    const float cc0 = T[c0 + dch], cc1 = T[c0 + dch + 1];
-                    const float d0 = T[c0 + dmh + dch], d1 = T[c0 + dmh + dch + 1];
+                    const auto ta = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, 0, 0);
+                    const auto tb = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, so_m, 0);
+                    const auto tc = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, so_c, 0);
+                    const auto td = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, so_mc, 0);
+                    const float a0 = __uint_as_float((uint32_t)ta[0]), a1 = __uint_as_float((uint32_t)ta[1]);
+                    const float b0v = __uint_as_float((uint32_t)tb[0]), b1 = __uint_as_float((uint32_t)tb[1]);
+                    const float cc0 = __uint_as_float((uint32_t)tc[0]), cc1 = __uint_as_float((uint32_t)tc[1]);
+                    const float d0 = __uint_as_float((uint32_t)td[0]), d1 = __uint_as_float((uint32_t)td[1]);
                     float acc4 = w00 * __builtin_fmaf(fr, a1 - a0, a0);
                     acc4 = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc4);
                     acc4 = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc4);
