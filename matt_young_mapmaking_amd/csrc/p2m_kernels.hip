@@ -1168,6 +1168,7 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
     __shared__ unsigned long long fb_base;
     /* inner edge of the deferral band, relative to the outer one */
     const float lo_band = (float)((1.0 - (double)A.band_c) / (1.0 + (double)A.band_c));
+    const float u_zero = A.u_max + A.band, u_edge = A.u_max - A.band, half_in = 0.5f - A.band;
 
     const int nrt = A.g.nrt;
     /* the table slices as raw buffers: a 32-bit byte offset per pixel, the
@@ -1255,10 +1256,13 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 const float u = __builtin_amdgcn_logf(r2) * A.u_k1 - A.u_k0;
                 const float fl = __builtin_floorf(u);
                 const float fu = u - fl;
-                const bool zero = u > A.u_max + A.band;           /* r > rmaxt: value 0 */
-                const bool defer = c2 > thr_hi * lo_band || !(dd > 1e-3f) ||
-                                   (!zero && ((u > -A.band && u < A.band) || u > A.u_max - A.band ||
-                                              (u >= 0.0f && (fu < A.band || fu > 1.0f - A.band))));
+                const bool zero = u > u_zero;                     /* r > rmaxt: value 0 */
+                /* deferred: near the disc edge, a degenerate chord, or u near a
+                 * cell boundary (|fu - 1/2| > 1/2 - band), 0, or the r cut-off;
+                 * bitwise, so the test compiles to compares and mask logic */
+                const bool defer = (c2 > thr_hi * lo_band) | !(dd > 1e-3f) |
+                                   (!zero & ((__builtin_fabsf(u) < A.band) | (u > u_edge) |
+                                             ((u >= 0.0f) & (__builtin_fabsf(fu - 0.5f) > half_in))));
                 if (defer) {
                     const uint32_t slot = atomicAdd(&sl->nfb, 1u);
                     const uint4 e = make_uint4((uint32_t)G.ring, (uint32_t)ip, sl->halo[s], 0u);
@@ -1271,8 +1275,8 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                     continue;
                 }
                 const bool clampr = u < 0.0f;                     /* r < rmint: r = rmint + 1e-5 */
-                int ir = clampr ? 1 : (int)fl + 1;
-                ir = ir < 1 ? 1 : (ir > nrt - 1 ? nrt - 1 : ir);
+                /* floor(u) + 1 clamped to [1, nrt-1] (u < 0 gives 1) */
+                const int ir = __builtin_amdgcn_fmed3f(fl + 1.0f, 1.0f, (float)(nrt - 1));
                 const float fr = clampr ? A.fr_clamp
                                         : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 const int vo = (sl->colbase[s] + (ir - 1)) * 4;
