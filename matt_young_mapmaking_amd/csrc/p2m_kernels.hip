@@ -1139,14 +1139,14 @@ __device__ static inline float sin2f(float x)
     return sn * sn;
 }
 
-/* ln(1+q) for 0 <= q < 0.08 (2 atanh(q/(2+q))) with the hardware reciprocal */
-__device__ static inline float log1p_small_rcp(float q)
+/* ln(1+q) / 2 for 0 <= q < 0.08 (atanh(q/(2+q))) with the hardware reciprocal */
+__device__ static inline float half_log1p_small_rcp(float q)
 {
     float s = q * __builtin_amdgcn_rcpf(2.0f + q);
     float s2 = s * s;
     float p = __builtin_fmaf(s2, 1.0f / 7.0f, 1.0f / 5.0f);
     p = __builtin_fmaf(s2, p, 1.0f / 3.0f);
-    return 2.0f * __builtin_fmaf(s * s2, p, s);
+    return __builtin_fmaf(s * s2, p, s);
 }
 
 template <int NM>
@@ -1277,8 +1277,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 const bool clampr = u < 0.0f;                     /* r < rmint: r = rmint + 1e-5 */
                 /* floor(u) + 1 clamped to [1, nrt-1] (u < 0 gives 1) */
                 const int ir = __builtin_amdgcn_fmed3f(fl + 1.0f, 1.0f, (float)(nrt - 1));
-                const float fr = clampr ? A.fr_clamp
-                                        : 0.5f * log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
+                /* both sides evaluated, then selected: no branch in the loop */
+                const float fr_in = half_log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
+                const float fr = clampr ? A.fr_clamp : fr_in;
                 const int vo = (sl->colbase[s] + (ir - 1)) * 4;
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
                 n_upd++;
