@@ -1,9 +1,12 @@
-"""The enumeration of a ring run's tile pieces (p2m_pieces.h): the binning
-assigns segment slots with run_pieces and writes the first two pieces of
-every run with first_two_pieces, so the two must agree piece for piece.
-Checked on the host over every run start and length (strided on long
-rings), ring lengths that are and are not multiples of the tile width, and
-runs that wrap past the end of the ring."""
+"""The enumeration of a ring run's tile pieces (p2m_pieces.h), which the
+binning uses to cut every (halo, ring) run at the tile boundaries: each piece
+lies in one tile, the pieces follow the run from its first pixel in ring
+order, wrapping past the ring's end once, and cover it exactly; the emit
+pass writes every run's first piece with first_two_pieces, which must give
+run_pieces' first pieces.  Checked on
+the host over every run start and length (strided on long rings), ring
+lengths that are and are not multiples of the tile width, and tile widths
+64..2048."""
 import os
 import shutil
 import subprocess
@@ -14,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "matt_young_mapmaking_amd", "csrc")
 
 
-def test_first_two_pieces_match_run_pieces(tmp_path):
+def test_run_pieces_partition_the_run(tmp_path):
     hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
     if not hipcc:
         pytest.skip("hipcc not found")
