@@ -68,6 +68,7 @@ struct p2m_ctx {
     TabGeom g;
     float *d_flat[P2M_MAXMAPS] = {nullptr};
     float *d_prof[P2M_MAXMAPS] = {nullptr};    /* the profile's slice, zero-padded (FP32 painter) */
+    float *d_profq = nullptr;                   /* the slices interleaved map by map (FP32 painter) */
     int64_t n_flat = 0, n_prof = 0;
     double *d_inv_node2 = nullptr;
     float *d_inv_node2f = nullptr;
@@ -290,6 +291,7 @@ void p2m_destroy(p2m_ctx *c)
     for (int m = 0; m < P2M_MAXMAPS; ++m) {
         dfree(c->d_hp64[m]); dfree(c->d_hp32[m]); dfree(c->d_fs64[m]); dfree(c->d_flat[m]); dfree(c->d_prof[m]);
     }
+    dfree(c->d_profq);
     dfree(c->d_count); dfree(c->d_hash); dfree(c->d_fs_count); dfree(c->d_fs_hash);
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_pos); dfree(c->d_ovf); dfree(c->d_ctr);
@@ -437,6 +439,10 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
         for (int64_t i = 0; i < nt / 2; ++i) prof[(size_t)i] = table[(p - 1) + 2 * i];
         if (!c->d_prof[slot] && dmalloc((void **)&c->d_prof[slot], sizeof(float) * c->n_prof)) return P2M_E_NOMEM;
         HIPCHK(hipMemcpy(c->d_prof[slot], prof.data(), sizeof(float) * c->n_prof, hipMemcpyHostToDevice));
+        if (!c->d_profq && dmalloc((void **)&c->d_profq, sizeof(float) * c->n_prof * c->cfg.n_maps)) return P2M_E_NOMEM;
+        p2m_launch_interleave(c->d_prof[slot], c->n_prof, c->cfg.n_maps, slot, c->d_profq, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     /* FP32 path constants: node radii, ln r -> cell coordinate */
     std::vector<double> inv(g.nrt + 2);
@@ -721,10 +727,10 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
         t.band_c = 4e-6f;
         t.inv_node2f = c->d_inv_node2f;
-        t.n_prof = c->n_prof;
+        t.n_prof = c->n_prof; t.tab_q = c->d_profq;
         t.fb = c->d_fb; t.fb_count = pc + 6; t.fb_cap = c->fb_cap;
         for (int m = 0; m < c->cfg.n_maps; ++m) {
-            t.tab_flat[m] = c->d_flat[m]; t.tab_prof[m] = c->d_prof[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
+            t.tab_flat[m] = c->d_flat[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
         }
         t.count = c->d_count; t.hash = c->d_hash; t.counters = pc;
         /* persistent grid of exactly the resident workgroups (a multiple

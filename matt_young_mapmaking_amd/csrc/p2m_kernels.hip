@@ -1149,6 +1149,27 @@ __device__ static inline float half_log1p_small_rcp(float q)
     return __builtin_fmaf(s * s2, p, s);
 }
 
+/* NF consecutive floats of a raw buffer at byte offset vo + so, in loads of
+ * 16 and 8 bytes (dword alignment is enough for buffer loads) */
+template <int NF>
+__device__ static inline void load_floats(__amdgpu_buffer_rsrc_t r, int vo, int so, float *f)
+{
+    int k = 0;
+#pragma unroll
+    for (; k + 4 <= NF; k += 4) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, vo + 4 * k, so, 0);
+        f[k] = __uint_as_float((uint32_t)v[0]); f[k + 1] = __uint_as_float((uint32_t)v[1]);
+        f[k + 2] = __uint_as_float((uint32_t)v[2]); f[k + 3] = __uint_as_float((uint32_t)v[3]);
+    }
+#pragma unroll
+    for (; k + 2 <= NF; k += 2) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, vo + 4 * k, so, 0);
+        f[k] = __uint_as_float((uint32_t)v[0]); f[k + 1] = __uint_as_float((uint32_t)v[1]);
+    }
+#pragma unroll
+    for (; k < NF; ++k) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo + 4 * k, so, 0));
+}
+
 template <int NM>
 __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *__restrict__ tdesc,
                                                    const unsigned long long *__restrict__ toff)
@@ -1173,10 +1194,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
     const int nrt = A.g.nrt;
     /* the table slices as raw buffers: a 32-bit byte offset per pixel, the
      * corner strides as scalar offsets, reads past the slice return 0 */
-    __amdgpu_buffer_rsrc_t trs[NM];
-    for (int m = 0; m < NM; ++m)
-        trs[m] = __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_prof[m], (short)0, (int)(A.n_prof * 4), 0x00020000);
-    const int so_m = nrt * 4, so_c = nrt * A.g.nmht * 4, so_mc = so_m + so_c;
+    const __amdgpu_buffer_rsrc_t trs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_q, (short)0, (int)(A.n_prof * NM * 4), 0x00020000);
+    const int so_m = nrt * NM * 4, so_c = nrt * A.g.nmht * NM * 4, so_mc = so_m + so_c;
     const int tid = threadIdx.x;
     unsigned long long n_upd = 0, n_fb = 0;
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
@@ -1280,18 +1300,21 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 /* both sides evaluated, then selected: no branch in the loop */
                 const float fr_in = half_log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 const float fr = clampr ? A.fr_clamp : fr_in;
-                const int vo = (sl->colbase[s] + (ir - 1)) * 4;
+                /* the 2 NM floats of each corner column: every map at nodes ir-1, ir */
+                const int vo = (sl->colbase[s] + (ir - 1)) * (4 * NM);
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
                 n_upd++;
+                float ta[2 * NM], tb[2 * NM], tc[2 * NM], td[2 * NM];
+                load_floats<2 * NM>(trs, vo, 0, ta);
+                load_floats<2 * NM>(trs, vo, so_m, tb);
+                load_floats<2 * NM>(trs, vo, so_c, tc);
+                load_floats<2 * NM>(trs, vo, so_mc, td);
+#pragma unroll
                 for (int m = 0; m < NM; ++m) {
-                    const auto ta = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, 0, 0);
-                    const auto tb = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, so_m, 0);
-                    const auto tc = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, so_c, 0);
-                    const auto td = __builtin_amdgcn_raw_buffer_load_b64(trs[m], vo, so_mc, 0);
-                    const float a0 = __uint_as_float((uint32_t)ta[0]), a1 = __uint_as_float((uint32_t)ta[1]);
-                    const float b0v = __uint_as_float((uint32_t)tb[0]), b1 = __uint_as_float((uint32_t)tb[1]);
-                    const float cc0 = __uint_as_float((uint32_t)tc[0]), cc1 = __uint_as_float((uint32_t)tc[1]);
-                    const float d0 = __uint_as_float((uint32_t)td[0]), d1 = __uint_as_float((uint32_t)td[1]);
+                    const float a0 = ta[m], a1 = ta[NM + m];
+                    const float b0v = tb[m], b1 = tb[NM + m];
+                    const float cc0 = tc[m], cc1 = tc[NM + m];
+                    const float d0 = td[m], d1 = td[NM + m];
                     float acc4 = w00 * __builtin_fmaf(fr, a1 - a0, a0);
                     acc4 = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc4);
                     acc4 = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc4);
@@ -1405,6 +1428,22 @@ int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)
     if (g > 64) g = 64;
     hipLaunchKernelGGL(k_fallback, dim3((unsigned)g, P2M_SHARDS), dim3(256), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* q[c * n_maps + slot] = slice[c]: one map's profile slice into the
+ * interleaved table of the FP32 painter */
+__global__ void k_interleave(const float *slice, int64_t n, int32_t n_maps, int32_t slot, float *q)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        q[c * n_maps + slot] = slice[c];
+}
+
+void p2m_launch_interleave(const float *slice, int64_t n, int32_t n_maps, int32_t slot, float *q, hipStream_t s)
+{
+    if (n <= 0) return;
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_interleave, dim3((unsigned)g), dim3(256), 0, s, slice, n, n_maps, slot, q);
 }
 
 /* widest tile the painters take: a segment (at most one tile wide) must fit
