@@ -321,7 +321,8 @@ int p2m_clear(p2m_ctx *c)
     if (c->d_fs_count) p2m_launch_fill_u32(c->d_fs_count, c->fsnpix, 0u, c->stream);
     if (c->d_fs_hash) p2m_launch_fill_u64(c->d_fs_hash, c->fsnpix, c->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
+    /* no wait: later work on the painter's stream is ordered after the
+     * fills, and every readback waits for the stream first */
     c->hp_zero = true;
     c->hp_valid = !tiled;
     return 0;
@@ -330,7 +331,7 @@ int p2m_clear(p2m_ctx *c)
 /* Zero-fill the HEALPix arrays if a lazy clear left them unwritten. */
 static int ensure_hp_valid(p2m_ctx *c)
 {
-    if (c->hp_valid) return 0;
+    if (c->hp_valid) { HIPCHK(hipStreamSynchronize(c->stream)); return 0; }
     for (int m = 0; m < c->cfg.n_maps; ++m) {
         if (c->d_hp64[m]) p2m_launch_fill_f64(c->d_hp64[m], c->hpnpix, c->stream);
         if (c->d_hp32[m]) p2m_launch_fill_f32(c->d_hp32[m], c->hpnpix, c->stream);
