@@ -83,7 +83,7 @@ typedef struct {
     double  cut_low, cut_high;
     /* device time of the last p2m_paint* call: total, decode, binning (spatial
      * sort to segment lists), healpix painting, flat-sky painting (the tiled
-     * modes run it inside the binning's host wait; ms_bin excludes it) */
+     * modes run it on a second stream beside the binning) */
     double  ms_total, ms_decode, ms_bin, ms_paint, ms_finish;
     int32_t tile_w, tile_wg;  /* tile shape used (autotuned or set) */
     double  ms_tile;          /* the tile painter kernel alone, within ms_paint (0 for plain FP64) */

@@ -55,6 +55,8 @@ struct p2m_ctx {
     p2m_config cfg;
     hipStream_t stream = nullptr;
     hipEvent_t ev[10];
+    hipStream_t fs_stream = nullptr;            /* the flat-sky painter, beside the binning */
+    hipEvent_t ev_dec = nullptr, ev_fs = nullptr;
     unsigned long long *h_pin = nullptr;   /* pinned host words for mid-call readbacks */
     bool have_run = false;
     bool have_table[P2M_MAXMAPS] = {false};
@@ -112,6 +114,8 @@ struct p2m_ctx {
     unsigned long long *d_tile_off = nullptr;
     void *d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
+    void *d_scan_tmp_fs = nullptr;              /* the flat-sky painter's own scan storage */
+    size_t scan_tmp_fs_bytes = 0;
     uint2 *d_segs = nullptr;
     int64_t seg_cap = 0;
     float *d_stage = nullptr;
@@ -250,6 +254,16 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
     if (c->cfg.chunk_halos > (1LL << 30)) c->cfg.chunk_halos = 1LL << 30;   /* chunk indices are 32-bit */
     memset(&c->st, 0, sizeof(c->st));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return P2M_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->fs_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_dec, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fs, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (c->fs_stream) (void)hipStreamDestroy(c->fs_stream);
+        if (c->ev_dec) (void)hipEventDestroy(c->ev_dec);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return P2M_E_HIP;
+    }
     for (int i = 0; i < 10; ++i) (void)hipEventCreate(&c->ev[i]);
     if (hipHostMalloc((void **)&c->h_pin, sizeof(unsigned long long) * (Ctr::ALL + 8), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
@@ -288,6 +302,7 @@ void p2m_destroy(p2m_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->fs_stream) (void)hipStreamSynchronize(c->fs_stream);
     for (int m = 0; m < P2M_MAXMAPS; ++m) {
         dfree(c->d_hp64[m]); dfree(c->d_hp32[m]); dfree(c->d_fs64[m]); dfree(c->d_flat[m]); dfree(c->d_prof[m]);
     }
@@ -295,12 +310,15 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_count); dfree(c->d_hash); dfree(c->d_fs_count); dfree(c->d_fs_hash);
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_pos); dfree(c->d_ovf); dfree(c->d_ctr);
-    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_off); dfree(c->d_scan_tmp);
+    dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_off); dfree(c->d_scan_tmp); dfree(c->d_scan_tmp_fs);
     dfree(c->d_tile_ring); dfree(c->d_tile_desc); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
     dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
     dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
     for (int i = 0; i < 10; ++i) (void)hipEventDestroy(c->ev[i]);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
+    if (c->ev_dec) (void)hipEventDestroy(c->ev_dec);
+    if (c->ev_fs) (void)hipEventDestroy(c->ev_fs);
+    if (c->fs_stream) (void)hipStreamDestroy(c->fs_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -486,18 +504,24 @@ int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
 
 /* ------------------------------------------------------------------ paint */
 /* exclusive scan of n uint32 counts into uint64 offsets, growing the temp storage */
-static int scan_counts(p2m_ctx *c, const uint32_t *cnt, unsigned long long *off, int64_t n)
+static int scan_counts_on(const uint32_t *cnt, unsigned long long *off, int64_t n, void **tmp, size_t *tmp_bytes,
+                          hipStream_t s)
 {
     size_t need = 0;
-    if (p2m_scan_u32(cnt, off, n, nullptr, &need, c->stream)) return P2M_E_HIP;
-    if (need > c->scan_tmp_bytes || !c->d_scan_tmp) {
-        dfree(c->d_scan_tmp); c->d_scan_tmp = nullptr; c->scan_tmp_bytes = 0;
-        if (dmalloc(&c->d_scan_tmp, need + 256)) return P2M_E_NOMEM;
-        c->scan_tmp_bytes = need + 256;
+    if (p2m_scan_u32(cnt, off, n, nullptr, &need, s)) return P2M_E_HIP;
+    if (need > *tmp_bytes || !*tmp) {
+        dfree(*tmp); *tmp = nullptr; *tmp_bytes = 0;
+        if (dmalloc(tmp, need + 256)) return P2M_E_NOMEM;
+        *tmp_bytes = need + 256;
     }
-    size_t bytes = c->scan_tmp_bytes;
-    if (p2m_scan_u32(cnt, off, n, c->d_scan_tmp, &bytes, c->stream)) return P2M_E_HIP;
+    size_t bytes = *tmp_bytes;
+    if (p2m_scan_u32(cnt, off, n, *tmp, &bytes, s)) return P2M_E_HIP;
     return 0;
+}
+
+static int scan_counts(p2m_ctx *c, const uint32_t *cnt, unsigned long long *off, int64_t n)
+{
+    return scan_counts_on(cnt, off, n, &c->d_scan_tmp, &c->scan_tmp_bytes, c->stream);
 }
 
 /* The per-halo buffers exist after the first paint call even for an empty
@@ -577,7 +601,7 @@ static int decode_chunk(p2m_ctx *c, const float *d_src, int64_t n, int64_t base)
 
 static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long long n_items);
 
-static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat);
+static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat, hipStream_t s);
 
 static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long long n_items, bool allow_tune)
 {
@@ -608,6 +632,12 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         return 0;
     }
     if (c->cfg.nside > 0) {
+        /* the flat-sky painter needs only the decoded records: it runs on
+         * its own stream beside the binning and the tile painter */
+        HIPCHK(hipEventRecord(c->ev_dec, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->fs_stream, c->ev_dec, 0));
+        { int rc = launch_flat(c, pa, n_flat, c->fs_stream); if (rc) return rc; }
+        HIPCHK(hipEventRecord(c->ev_fs, c->fs_stream));
         if (!c->tuned) {
             if (allow_tune) { int rc = autotune(c, n_halo, n_flat, n_items); if (rc) return rc; }
             else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
@@ -664,9 +694,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_seg_count(b, c->stream);
         { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
-        /* segment and listed-pixel counts back to the host; the flat-sky
-         * painter (independent of the binning) is queued behind the copies
-         * so the device works while the host waits for them */
+        /* segment and listed-pixel counts back to the host (pinned memory) */
         unsigned long long nseg = 0, nlisted = 0;
         {
             unsigned long long hbuf[Ctr::ALL + 1];
@@ -675,7 +703,6 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             HIPCHK(hipMemcpyAsync(hb + Ctr::ALL, c->d_tile_off + c->n_tiles, sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipEventRecord(c->ev[6], c->stream));
-            { int rc = launch_flat(c, pa, n_flat); if (rc) return rc; }
             HIPCHK(hipEventSynchronize(c->ev[6]));
             const unsigned long long n_ovf = hb[Ctr::PAINT + Ctr::OVF];
             if ((int64_t)n_ovf > c->ovf_cap) {
@@ -759,20 +786,22 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             if (p2m_launch_fallback(f, (int64_t)(nlisted / 256) + 1024, c->stream)) return P2M_E_HIP;
         }
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->ev_fs, 0));
     } else {
         HIPCHK(hipEventRecord(c->ev[3], c->stream));
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
-        { int rc = launch_flat(c, pa, n_flat); if (rc) return rc; }
+        { int rc = launch_flat(c, pa, n_flat, c->stream); if (rc) return rc; }
     }
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     c->hp_zero = false;
     return 0;
 }
 
-/* flat-sky painter over the flattened box pixels, between events 8 and 9 */
-static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat)
+/* flat-sky painter over the flattened box pixels on stream s, between
+ * events 8 and 9 */
+static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat, hipStream_t s)
 {
-    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(hipEventRecord(c->ev[8], s));
     if (c->cfg.npix_fs > 0 && n_flat > 0) {
         if (n_flat + 1 > c->fs_cap || !c->d_fs_cnt) {
             dfree(c->d_fs_cnt); dfree(c->d_fs_off); c->d_fs_cnt = nullptr; c->d_fs_off = nullptr; c->fs_cap = 0;
@@ -782,12 +811,18 @@ static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat)
             c->fs_cap = cap;
         }
         pa.fs_cnt = c->d_fs_cnt; pa.fs_off = c->d_fs_off;
-        p2m_launch_fs_counts(pa, c->stream);
-        { int rc = scan_counts(c, c->d_fs_cnt, c->d_fs_off, n_flat + 1); if (rc) return rc; }
-        p2m_launch_fs_paint(pa, c->stream);
+        p2m_launch_fs_counts(pa, s);
+        {
+            /* its own scan storage when beside the binning */
+            const bool own = s != c->stream;
+            int rc = scan_counts_on(c->d_fs_cnt, c->d_fs_off, n_flat + 1, own ? &c->d_scan_tmp_fs : &c->d_scan_tmp,
+                                    own ? &c->scan_tmp_fs_bytes : &c->scan_tmp_bytes, s);
+            if (rc) return rc;
+        }
+        p2m_launch_fs_paint(pa, s);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[9], c->stream));
+    HIPCHK(hipEventRecord(c->ev[9], s));
     return 0;
 }
 
@@ -822,10 +857,10 @@ static int finish_stats(p2m_ctx *c)
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); s.ms_decode = ms;
     float ms_fs = 0;
     if (c->cfg.precision != P2M_PREC_PLAIN_FP64) (void)hipEventElapsedTime(&ms_fs, c->ev[8], c->ev[9]);
+    /* the flat-sky painter of the tiled modes runs on its own stream beside
+     * the binning: its time is reported as the finish phase */
     (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-    /* the flat-sky painter of the tiled modes runs inside the binning's
-     * host-synchronisation gap: its time is reported as the finish phase */
-    s.ms_bin = (c->cfg.nside > 0 && c->cfg.precision != P2M_PREC_PLAIN_FP64) ? std::max(0.0f, ms - ms_fs) : ms;
+    s.ms_bin = ms;
     (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); s.ms_paint = ms;
     if (c->cfg.precision == P2M_PREC_PLAIN_FP64) { (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]); s.ms_finish = ms; }
     else s.ms_finish = ms_fs;
