@@ -35,7 +35,7 @@ struct DecodeArgs {
     unsigned long long *n_flat_out;
     uint32_t *keys, *vals;       /* spatial sort key and slot of each record (tiled painters) */
     int32_t nphi;                /* azimuth bins of the key */
-    uint32_t key_max;            /* key of inert records (sorted last) */
+    uint32_t key_max;            /* the largest key, also given to inert records */
     unsigned long long *counters;/* sharded: [0] passed radial cut, [2] radius errors, [3] (halo, ring) items */
 };
 

@@ -812,10 +812,13 @@ __global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const u
     }
 }
 
+/* the largest key: the last azimuth bin of the last band, which the inert
+ * records share (their place in the order does not matter), so that the
+ * keys of nside 4096 fit 16 bits (two radix passes) */
 uint32_t p2m_key_max(int32_t nside, int32_t nphi)
 {
     const uint32_t nband = (uint32_t)((4 * nside - 2) >> 3) + 1;
-    return nband * (uint32_t)nphi;
+    return nband * (uint32_t)nphi - 1;
 }
 
 /* Sorts the records by the key k_setup wrote (stable radix sort of
