@@ -444,6 +444,10 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     g.lr0 = p2m_log(g.rmint); g.lc0 = p2m_log(g.chimint); g.lm0 = p2m_log(g.mhmint);
     /* padding covers every index interpolate_table can form past the end (Q6) */
     int64_t pad = 2LL * g.nrt * (g.nmht + 2) + 4LL * g.nrt + 64;
+    /* the painters read the tables through buffer resources with 32-bit byte
+     * offsets: one padded table, and the interleaved profile tables of all
+     * slots, must each stay below 2 GiB */
+    if ((nt + pad) * 4 > INT32_MAX || (nt + pad) / 2 * 4 * c->cfg.n_maps > INT32_MAX) return P2M_E_SIZE;
     c->n_flat = nt + pad;
     if (!c->d_flat[slot] && dmalloc((void **)&c->d_flat[slot], sizeof(float) * c->n_flat)) return P2M_E_NOMEM;
     std::vector<float> flat(c->n_flat, 0.f);

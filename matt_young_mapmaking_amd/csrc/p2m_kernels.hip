@@ -1011,11 +1011,46 @@ __device__ static inline int batch_prepare(BatchLDS *bl, uint32_t len, int nstag
 /* tile (ds_add_f64) and reduced to the FP64 map.                        */
 /* ------------------------------------------------------------------ */
 
+/* the per-halo values the exact per-pixel code reads, staged per segment */
 struct ExactSeg {
     BatchLDS b;
-    uint32_t halo[FSEG];
+    double hx[FSEG], hy[FSEG], hz[FSEG], dth[FSEG], vh[FSEG], chir[FSEG], fc[FSEG], fm[FSEG];
+    unsigned long long hash[FSEG];
     uint32_t lo[FSEG];
+    int32_t imh[FSEG], ichi[FSEG], cout[FSEG];
 };
+
+/* interpolate_table (maptable.f90:176-216) on staged per-halo values, the
+ * same operations in the same order as interp_exact; the table through a
+ * raw buffer over its n_flat floats, whose out-of-range reads return 0 as
+ * tab_flat's bounds test does (the flat index is never negative: r >= rmint
+ * and the cells are clamped to >= 1) */
+__device__ static inline double tab_buf(__amdgpu_buffer_rsrc_t tr, const TabGeom &g, int p, int ir, int imh, int ichi)
+{
+    const int64_t idx = (int64_t)(p - 1) + 2 * ((int64_t)(ir - 1) + (int64_t)g.nrt * ((int64_t)(imh - 1) + (int64_t)g.nmht * (int64_t)(ichi - 1)));
+    return (double)__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tr, (int)(idx * 4), 0, 0));
+}
+
+__device__ static inline double interp_staged(int chi_out, double chi_r, double fc, double fm, int imh, int ichi,
+                                              const TabGeom &g, __amdgpu_buffer_rsrc_t tr, int profile, double theta)
+{
+    if (chi_out) return 0.0;
+    double r = chi_r * p2m_tan(theta);
+    if (r > g.rmaxt) return 0.0;
+    if (r < g.rmint) r = g.rmint + 1e-5;
+    double lr = p2m_log(r);
+    int ir = (int)((lr - g.lr0) / g.drt) + 1;
+    double fr = lr - (g.lr0 + (double)(ir - 1) * g.drt);
+    double v = tab_buf(tr, g, profile, ir, imh, ichi) * (1.0 - fr) * (1.0 - fc) * (1.0 - fm);
+    v = v + tab_buf(tr, g, profile, ir + 1, imh, ichi) * (fr) * (1.0 - fc) * (1.0 - fm);
+    v = v + tab_buf(tr, g, profile, ir, imh + 1, ichi) * (1.0 - fr) * (fc) * (1.0 - fm);
+    v = v + tab_buf(tr, g, profile, ir + 1, imh + 1, ichi) * (fr) * (fc) * (1.0 - fm);
+    v = v + tab_buf(tr, g, profile, ir, imh, ichi + 1) * (1.0 - fr) * (1.0 - fc) * (fm);
+    v = v + tab_buf(tr, g, profile, ir + 1, imh, ichi + 1) * (fr) * (1.0 - fc) * (fm);
+    v = v + tab_buf(tr, g, profile, ir, imh + 1, ichi + 1) * (1.0 - fr) * (fc) * (fm);
+    v = v + tab_buf(tr, g, profile, ir + 1, imh + 1, ichi + 1) * (fr) * (fc) * (fm);
+    return v;
+}
 
 template <int NM>
 __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
@@ -1032,6 +1067,9 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
     __shared__ unsigned long long lds16[16];
     const int tid = threadIdx.x;
     unsigned long long n_upd = 0;
+    __amdgpu_buffer_rsrc_t trs[NM];
+    for (int m = 0; m < NM; ++m)
+        trs[m] = __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_flat[m], (short)0, (int)(A.n_flat * 4), 0x00020000);
     int64_t tbeg, tend, tstep;
     tile_range(A, tbeg, tend, tstep);
     for (int64_t t = tbeg; t < tend; t += tstep) {
@@ -1048,9 +1086,13 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
             uint32_t len = 0;
             if (tid < nstage) {
                 const uint2 sg = A.segs[b0 + tid];
-                sl->halo[tid] = sg.x;
+                const HaloRec &h = A.halos[sg.x];
                 sl->lo[tid] = sg.y >> 16;
                 len = sg.y & 0xffffu;
+                sl->hx[tid] = h.hx; sl->hy[tid] = h.hy; sl->hz[tid] = h.hz; sl->dth[tid] = h.dtheta;
+                sl->vh[tid] = h.vh; sl->chir[tid] = h.chi_r; sl->fc[tid] = h.fc; sl->fm[tid] = h.fm;
+                sl->imh[tid] = h.imh; sl->ichi[tid] = h.ichi; sl->cout[tid] = h.chi_out;
+                if (A.count) sl->hash[tid] = halo_hash(h.cat);
             }
             uint32_t P;
             const int nb = batch_prepare(&sl->b, len, nstage, P);
@@ -1059,18 +1101,19 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
             for (uint32_t p = tid; p < P; p += blockDim.x) {
                 const int s = segb[p];
                 const int q = (int)(sl->lo[s] + (p - sl->b.pre[s]));
-                const HaloRec &h = A.halos[sl->halo[s]];
                 const double phi = (((double)(G.ip0 + q) + G.off) * HPX_HALFPI) / G.den;
                 double sn, cs;
                 p2m_sincos(phi, &sn, &cs);
-                const double angle = angdist(h.hx, h.hy, h.hz, G.sth * cs, G.sth * sn, G.zr);
-                if (!(angle < h.dtheta)) continue;
+                const double angle = angdist(sl->hx[s], sl->hy[s], sl->hz[s], G.sth * cs, G.sth * sn, G.zr);
+                if (!(angle < sl->dth[s])) continue;
                 n_upd++;
                 for (int m = 0; m < NM; ++m)
-                    atomicAdd(&acc[m * W + q], h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle));
+                    atomicAdd(&acc[m * W + q], sl->vh[s] * interp_staged(sl->cout[s], sl->chir[s], sl->fc[s], sl->fm[s],
+                                                                         sl->imh[s], sl->ichi[s], A.g, trs[m],
+                                                                         A.profile, angle));
                 if (A.count) {
                     atomicAdd(&lcount[q], 1u);
-                    atomicAdd(&lhash[q], (unsigned long long)halo_hash(h.cat));
+                    atomicAdd(&lhash[q], sl->hash[s]);
                 }
             }
         }
