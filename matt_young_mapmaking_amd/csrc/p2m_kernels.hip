@@ -786,29 +786,55 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 /* spatial order of the painted halos                                  */
 /* ------------------------------------------------------------------ */
 
+__device__ static inline double dbl_of(uint32_t lo, uint32_t hi)
+{
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* The fields read here lie in bytes [32, 112) and [128, 160) of the
+ * 176-B record.  They are fetched with seven 16-B loads issued together and
+ * one wait (field by field, the compiler put the loads of the FP32 fields
+ * behind branches on chi_out, a memory round trip each). */
+static_assert(offsetof(HaloRec, zz0) == 32 && offsetof(HaloRec, phi0) == 40 && offsetof(HaloRec, a) == 48 &&
+              offsetof(HaloRec, cosang) == 56 && offsetof(HaloRec, s0) == 64 && offsetof(HaloRec, thr) == 72 &&
+              offsetof(HaloRec, chi_r) == 80 && offsetof(HaloRec, fc) == 88 && offsetof(HaloRec, fm) == 96 &&
+              offsetof(HaloRec, vh) == 104 && offsetof(HaloRec, cat) == 128 && offsetof(HaloRec, ichi) == 136 &&
+              offsetof(HaloRec, imh) == 140 && offsetof(HaloRec, irmin) == 144 && offsetof(HaloRec, irmax) == 148 &&
+              offsetof(HaloRec, chi_out) == 152 && sizeof(HaloRec) == 176, "HaloRec layout read by k_gather_halos");
+
 __global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const uint32_t *order, int64_t n,
                                                       BinHalo *bin, TileHalo *hot, HaloRec *full, int32_t nrt, int32_t nmht)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const HaloRec &h = in[order[i]];
+        const uint32_t o = order[i];
+        const uint4 *src = reinterpret_cast<const uint4 *>(in + o);
+        const uint4 c2 = src[2], c3 = src[3], c4 = src[4], c5 = src[5], c6 = src[6], c8 = src[8], c9 = src[9];
+        const double zz0 = dbl_of(c2.x, c2.y), phi0 = dbl_of(c2.z, c2.w);
+        const double a = dbl_of(c3.x, c3.y), cosang = dbl_of(c3.z, c3.w);
+        const double s0 = dbl_of(c4.x, c4.y), thr = dbl_of(c4.z, c4.w);
+        const double chi_r = dbl_of(c5.x, c5.y), fc = dbl_of(c5.z, c5.w);
+        const double fm = dbl_of(c6.x, c6.y), vh = dbl_of(c6.z, c6.w);
+        const int64_t cat = (int64_t)(((unsigned long long)c8.y << 32) | c8.x);
+        const int32_t ichi = (int32_t)c8.z, imh = (int32_t)c8.w;
+        const int32_t irmin = (int32_t)c9.x, irmax = (int32_t)c9.y, chi_out = (int32_t)c9.z;
         BinHalo b;
-        b.cosang = h.cosang; b.zz0 = h.zz0; b.a = h.a; b.phi0 = h.phi0;
-        b.irmin = h.irmin; b.irmax = h.irmax;
+        b.cosang = cosang; b.zz0 = zz0; b.a = a; b.phi0 = phi0;
+        b.irmin = irmin; b.irmax = irmax;
         bin[i] = b;
         if (hot) {
             TileHalo t;
-            t.zz0 = h.zz0; t.s0 = h.s0; t.phi0 = h.phi0; t.thr = h.thr;
-            t.chi2 = (float)(h.chi_r * h.chi_r);
-            t.vh = h.chi_out ? 0.0f : (float)h.vh;
-            t.fc = (float)h.fc; t.fm = (float)h.fm;
+            t.zz0 = zz0; t.s0 = s0; t.phi0 = phi0; t.thr = thr;
+            t.chi2 = (float)(chi_r * chi_r);
+            t.vh = chi_out ? 0.0f : (float)vh;
+            t.fc = (float)fc; t.fm = (float)fm;
             /* a halo beyond the table's chi range paints zeros (vh = 0); its
              * cell indices may lie outside the table, so point it at column 0 */
-            t.colbase = h.chi_out ? 0 : nrt * ((h.imh - 1) + nmht * (h.ichi - 1));
+            t.colbase = chi_out ? 0 : nrt * ((imh - 1) + nmht * (ichi - 1));
             t.pad = 0;
-            t.hash = halo_hash(h.cat);
+            t.hash = halo_hash(cat);
             hot[i] = t;
         }
-        if (full) full[i] = h;
+        if (full) full[i] = in[o];
     }
 }
 
