@@ -25,6 +25,24 @@
 /* helpers                                                                */
 /* ===================================================================== */
 
+__device__ static inline double dbl_of(uint32_t lo, uint32_t hi)
+{
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* Loaded words that the code below needs on every path: an empty asm that
+ * uses them, so that the compiler issues their loads together here instead
+ * of moving each behind the branch that first uses it (a memory round trip
+ * per branch). */
+__device__ static inline void pin4(uint4 v)
+{
+    asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+}
+__device__ static inline void pin2(uint2 v)
+{
+    asm volatile("" ::"v"(v.x), "v"(v.y));
+}
+
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -510,6 +528,10 @@ __global__ void __launch_bounds__(256) k_ring_counts(BinArgs B)
  * per (chunk, tile) reserves the slots of all its pieces.  A piece whose
  * tile finds no hash slot within BIN_PROBE probes takes one global atomic
  * of its own. */
+static_assert(offsetof(BinHalo, cosang) == 0 && offsetof(BinHalo, zz0) == 8 && offsetof(BinHalo, a) == 16 &&
+              offsetof(BinHalo, phi0) == 24 && offsetof(BinHalo, irmin) == 32 && offsetof(BinHalo, irmax) == 36 &&
+              sizeof(BinHalo) == 40, "BinHalo layout read by k_seg_count");
+
 #define BIN_CH 1024
 #define BIN_HASH 1024
 #define BIN_HASH_BITS 10
@@ -643,7 +665,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         unsigned long long a = c0 + wv * per, e = a + per;
         if (e > B.n_items) e = B.n_items;
         wave_range(B.ring_off, B.group_halo, B.n_halo, a, e, [&](unsigned long long item, int64_t hi, unsigned long long k) {
-            const BinHalo &h = B.halos[hi];
+            const uint2 *hs = reinterpret_cast<const uint2 *>(B.halos + hi);
+            const uint2 q0 = hs[0], q1 = hs[1], q2 = hs[2], q3 = hs[3], q4 = hs[4];
+            pin2(q0); pin2(q1); pin2(q2); pin2(q3); pin2(q4);
+            BinHalo h;
+            h.cosang = dbl_of(q0.x, q0.y); h.zz0 = dbl_of(q1.x, q1.y);
+            h.a = dbl_of(q2.x, q2.y); h.phi0 = dbl_of(q3.x, q3.y);
+            h.irmin = (int32_t)q4.x; h.irmax = (int32_t)q4.y;
             const int32_t ring = h.irmin + (int32_t)k;
             int32_t lo = 0, np = 0;
             int32_t npr, kshift; int64_t first;
@@ -786,15 +814,12 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 /* spatial order of the painted halos                                  */
 /* ------------------------------------------------------------------ */
 
-__device__ static inline double dbl_of(uint32_t lo, uint32_t hi)
-{
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 /* The fields read here lie in bytes [32, 112) and [128, 160) of the
  * 176-B record.  They are fetched with seven 16-B loads issued together and
  * one wait (field by field, the compiler put the loads of the FP32 fields
  * behind branches on chi_out, a memory round trip each). */
+static_assert(offsetof(HaloRec, hx) == 0 && offsetof(HaloRec, hy) == 8 && offsetof(HaloRec, hz) == 16 &&
+              offsetof(HaloRec, dtheta) == 24, "HaloRec layout read by k_fallback");
 static_assert(offsetof(HaloRec, zz0) == 32 && offsetof(HaloRec, phi0) == 40 && offsetof(HaloRec, a) == 48 &&
               offsetof(HaloRec, cosang) == 56 && offsetof(HaloRec, s0) == 64 && offsetof(HaloRec, thr) == 72 &&
               offsetof(HaloRec, chi_r) == 80 && offsetof(HaloRec, fc) == 88 && offsetof(HaloRec, fm) == 96 &&
@@ -1468,7 +1493,10 @@ __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
          i += (unsigned long long)gridDim.x * blockDim.x) {
         const uint4 e = fbr[i];
         const int32_t ring = (int32_t)e.x, ip = (int32_t)e.y;
-        const HaloRec &h = A.halos[A.order[e.z]];
+        /* bytes [0, 32), [80, 112) and [128, 160) of the record: the fields
+         * of the exact test and value, requested before the ring geometry */
+        const uint4 *hs = reinterpret_cast<const uint4 *>(A.halos + A.order[e.z]);
+        const uint4 c0 = hs[0], c1 = hs[1], c5 = hs[5], c6 = hs[6], c8 = hs[8], c9 = hs[9];
         int32_t npr, kshift; int64_t first;
         ring_info(A.nside, ring, npr, kshift, first);
         double zr, sth, off, den;
@@ -1477,17 +1505,28 @@ __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
         const double phi = (((double)ip + off) * HPX_HALFPI) / den;
         double sn, cs;
         p2m_sincos(phi, &sn, &cs);
-        const double angle = angdist(h.hx, h.hy, h.hz, sth * cs, sth * sn, zr);
-        if (!(angle < h.dtheta)) continue;
+        pin4(c0); pin4(c1); pin4(c5); pin4(c6); pin4(c8); pin4(c9);
+        const double hx = dbl_of(c0.x, c0.y), hy = dbl_of(c0.z, c0.w), hz = dbl_of(c1.x, c1.y);
+        const double dtheta = dbl_of(c1.z, c1.w);
+        const double chi_r = dbl_of(c5.x, c5.y), fc = dbl_of(c5.z, c5.w), fm = dbl_of(c6.x, c6.y);
+        const double vh = dbl_of(c6.z, c6.w);
+        const int64_t cat = (int64_t)(((unsigned long long)c8.y << 32) | c8.x);
+        const int32_t ichi = (int32_t)c8.z, imh = (int32_t)c8.w, chi_out = (int32_t)c9.z;
+        const double angle = angdist(hx, hy, hz, sth * cs, sth * sn, zr);
+        if (!(angle < dtheta)) continue;
         n_upd++;
         const int64_t pix = first + ip;
         for (int m = 0; m < A.n_maps; ++m) {
-            double v = h.vh * interp_exact(h, A.g, A.tab_flat[m], A.n_flat, A.profile, angle);
+            /* built from the kernel argument inside the loop, so that the
+             * resource stays in scalar registers */
+            const __amdgpu_buffer_rsrc_t tr =
+                __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_flat[m], (short)0, (int)(A.n_flat * 4), 0x00020000);
+            double v = vh * interp_staged(chi_out, chi_r, fc, fm, imh, ichi, A.g, tr, A.profile, angle);
             atomicAdd(A.map32[m] + pix, (float)v);
         }
         if (A.count) {
             atomicAdd(&A.count[pix], 1u);
-            atomicAdd(&A.hash[pix], (unsigned long long)halo_hash(h.cat));
+            atomicAdd(&A.hash[pix], (unsigned long long)halo_hash(cat));
         }
     }
     block_count(&A.counters[1], n_upd, lds16);
