@@ -623,6 +623,13 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
     pa.count = c->d_count; pa.hash = c->d_hash; pa.fs_count = c->d_fs_count; pa.fs_hash = c->d_fs_hash;
     pa.counters = pc;
 
+    if (mode != P2M_PREC_PLAIN_FP64 && c->cfg.nside > 0 && !c->tuned) {
+        /* tile shape first: the tuner's trials paint and clear both maps, so
+         * this call's own painting (the flat-sky map included) starts after
+         * the last trial */
+        if (allow_tune) { int rc = autotune(c, n_halo, n_flat, n_items); if (rc) return rc; }
+        else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
+    }
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (mode == P2M_PREC_PLAIN_FP64) {
         if (c->cfg.nside > 0) p2m_launch_paint_plain_hp(pa, c->stream);
@@ -642,10 +649,6 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         HIPCHK(hipStreamWaitEvent(c->fs_stream, c->ev_dec, 0));
         { int rc = launch_flat(c, pa, n_flat, c->fs_stream); if (rc) return rc; }
         HIPCHK(hipEventRecord(c->ev_fs, c->fs_stream));
-        if (!c->tuned) {
-            if (allow_tune) { int rc = autotune(c, n_halo, n_flat, n_items); if (rc) return rc; }
-            else { int rc = build_tiles(c, 10); if (rc) return rc; c->wg = 256; c->tuned = true; }
-        }
         /* painted halos in spatial order (keys written by decode):
          * neighbouring halos share tiles */
         {
@@ -1024,8 +1027,17 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long lo
         int rc = build_tiles(c, cd[0]); if (rc) return rc;
         c->wg = cd[1]; c->tuned = true;
         c->hp_zero = true;
+        /* each trial starts from zero paint counters, as a call does: the
+         * deferred-pixel and overflow lists are appended at their counters */
+        p2m_launch_clear_ctr(c->d_ctr, Ctr::PAINT, Ctr::TOTAL - Ctr::PAINT, c->stream);
         rc = paint_halos(c, n_halo, n_flat, n_items, false);
-        if (rc == P2M_E_INVALID) { (void)p2m_clear(c); continue; }
+        if (rc == P2M_E_INVALID) {
+            /* a shape the painter rejects at launch: the trial's flat-sky
+             * paint is still on its stream, so the clear waits for it */
+            HIPCHK(hipStreamWaitEvent(c->stream, c->ev_fs, 0));
+            (void)p2m_clear(c);
+            continue;
+        }
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->stream));
         float ms = 0;

@@ -143,3 +143,46 @@ def test_chunking_tiles_and_repeatability(table):
         assert np.array_equal(hc, ref[1]) and np.array_equal(hh, ref[2])
         nz = ref[0] != 0
         assert np.max(np.abs(hp[nz] / ref[0][nz] - 1)) < 2e-7
+
+
+def test_autotuned_call_equals_fixed_shape(table):
+    """A context with no tile shape set tunes on its first large call: the
+    trials paint and clear both maps, then the call paints once.  Its maps,
+    membership and counters must equal those of a fixed-shape call, and a
+    second call must add the same catalogue again."""
+    h = catalog.config_catalog("C3", n=4_000_000)
+    out = []
+    for tile in (None, (1024, 256)):
+        with abi.Painter(nside=2048, npix_fs=512, precision=abi.PREC_FP32_COMP, membership=True) as p:
+            if tile:
+                p.set_tile(*tile)
+            p.set_run()
+            p.load_table(0, *table)
+            p.paint(h)
+            s = p.stats()
+            hp, fs = p.maps()
+            hc, hh, fc, fh = p.membership()
+            if tile is None:
+                p.paint(h)
+                hp2, fs2 = p.maps()
+                hc2, hh2, fc2, fh2 = p.membership()
+        out.append((s, hp, fs, hc, hh, fc, fh))
+    (s, hp, fs, hc, hh, fc, fh), (s0, hp0, fs0, hc0, hh0, fc0, fh0) = out
+    assert s["n_painted"] >= 200000               # large enough to tune
+    assert s["tile_w"] in (512, 1024, 2048) and s["tile_wg"] in (128, 256)
+    for k in ("n_painted", "n_listed_hp", "n_upd_hp", "n_upd_fs", "n_fallback"):
+        assert s[k] == s0[k], k
+    assert s0["n_upd_fs"] > 0
+    assert np.array_equal(hc, hc0) and np.array_equal(hh, hh0)
+    assert np.array_equal(fc, fc0) and np.array_equal(fh, fh0)
+    nz = hp0 != 0
+    assert np.count_nonzero((hp != 0) != nz) == 0
+    assert np.max(np.abs(hp[nz] / hp0[nz] - 1)) < 2e-7
+    nzf = fs0 != 0
+    assert np.count_nonzero(nzf) > 0 and np.count_nonzero((fs != 0) != nzf) == 0
+    assert np.max(np.abs(fs[nzf] / fs0[nzf] - 1)) <= 1e-12
+    # the second call accumulates onto the first with the tuned shape
+    assert np.array_equal(hc2, 2 * hc) and np.array_equal(fc2, 2 * fc)
+    assert np.array_equal(hh2, hh * np.uint64(2)) and np.array_equal(fh2, fh * np.uint64(2))
+    assert np.max(np.abs(hp2[nz] / (2 * hp[nz]) - 1)) < 4e-7
+    assert np.max(np.abs(fs2[nzf] / (2 * fs[nzf]) - 1)) <= 1e-12
