@@ -131,7 +131,11 @@ int p2m_get_stats(p2m_ctx *ctx, p2m_stats *st);
 /* Tile shape of the tiled painter: width in pixels along a ring (power of 2,
  * 64..2048) and workgroup size (a multiple of 64, 64..512; 64..256 for
  * P2M_PREC_FP32_COMP).
- * 0,0 = autotune on first paint. */
+ * 0,0 (and a context that never calls this) = the library picks the shape on
+ * the next paint call: with at least 2e5 halos past the cuts and cleared maps
+ * it times each candidate shape on those halos (painting, then clearing the
+ * maps) and keeps the fastest; otherwise, or with P2M_NO_AUTOTUNE set in the
+ * environment, 1024,256.  The shape then holds until the next call here. */
 int p2m_set_tile(p2m_ctx *ctx, int32_t tile_w, int32_t tile_wg);
 
 /* Products of pks2map.f90:321-357 computed on the device from map slot
