@@ -35,6 +35,8 @@ extern "C" {
 #define P2M_E_STATE      -6   /* call out of order (no cosmology / table loaded) */
 #define P2M_E_IO         -7   /* file could not be written */
 #define P2M_E_NODEVICE   -8   /* no GPU visible */
+#define P2M_E_INTERNAL   -9   /* a device-side consistency check failed (segment or tile out of range);
+                                 the maps of that call are not valid */
 
 /* precision / algorithm of the painter */
 #define P2M_PREC_PLAIN_FP64  0  /* one work item per halo, FP64 everything, FP64 global atomics (baseline) */
@@ -87,6 +89,13 @@ typedef struct {
     double  ms_total, ms_decode, ms_bin, ms_paint, ms_finish;
     int32_t tile_w, tile_wg;  /* tile shape used (autotuned or set) */
     double  ms_tile;          /* the tile painter kernel alone, within ms_paint (0 for plain FP64) */
+    /* catalogue handed over in host memory: bytes copied to the device, the
+     * copy stream's time from the call's start to its last copy, and whether
+     * every block was pinned (page-locked) memory */
+    int64_t h2d_bytes;
+    double  ms_h2d;
+    int32_t h2d_pinned;
+    int32_t n_redo_tiles;     /* tiles repainted after the deferred-pixel list grew (0 normally) */
 } p2m_stats;
 
 /* Context on one GPU; allocates the maps (zeroed).  Replaces the allocations
@@ -114,6 +123,29 @@ int p2m_load_table(p2m_ctx *ctx, int slot, const int32_t hdr_i[3], const float h
 int p2m_paint(p2m_ctx *ctx, const float *halos, int64_t n, int64_t index_base);
 /* Same, with the halos already in device memory (HBM-resident input). */
 int p2m_paint_device(p2m_ctx *ctx, const float *d_halos, int64_t n, int64_t index_base);
+
+/* Page-locked host memory for catalogue blocks (hipHostMalloc on the
+ * context's device).  Host catalogues in such memory reach the device by
+ * DMA overlapped with the decode, at the PCIe rate; pageable memory is
+ * staged by the runtime at a lower rate.  Replaces the allocate() of the
+ * halo arrays in loadpksc (pksc.f90:54-58).  p2m_host_free releases it. */
+int p2m_host_alloc(p2m_ctx *ctx, int64_t bytes, void **ptr);
+int p2m_host_free(p2m_ctx *ctx, void *ptr);
+
+/* Streamed catalogue pass: the block read of loadpksc (pksc.f90:51-66) fed
+ * to the halo loop (pks2map.f90:276-303) block by block.
+ *   p2m_stream_begin(ctx, n_total)   -- n_total: halos of all blocks together
+ *   p2m_stream_push(ctx, halos, n, index_base)   -- one block of 7-float records
+ *       (host memory, pinned or pageable); copied and decoded asynchronously.
+ *       The caller may overwrite a block's buffer once the NEXT push (or the
+ *       finish) has returned, so two buffers used in turn keep the copy
+ *       engine busy while the host reads the next block.
+ *   p2m_stream_finish(ctx)           -- bins and paints every pushed halo
+ * p2m_paint is begin + one push + finish.  Pushing more than n_total halos
+ * in total returns P2M_E_SIZE. */
+int p2m_stream_begin(p2m_ctx *ctx, int64_t n_total);
+int p2m_stream_push(p2m_ctx *ctx, const float *halos, int64_t n, int64_t index_base);
+int p2m_stream_finish(p2m_ctx *ctx);
 
 /* Zero the maps (and membership arrays). */
 int p2m_clear(p2m_ctx *ctx);

@@ -19,10 +19,11 @@ FLAG_MEMBERSHIP = 1
 RUN_PRECUT = 1
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "size mismatch", -3: "HIP error", -4: "allocation failed",
-          -5: "radius", -6: "state", -7: "io", -8: "no GPU"}
+          -5: "radius", -6: "state", -7: "io", -8: "no GPU", -9: "internal check"}
 
 EXPORTS = ["p2m_create", "p2m_destroy", "p2m_strerror", "p2m_set_run", "p2m_rofz", "p2m_zofr", "p2m_load_table",
-           "p2m_paint", "p2m_paint_device", "p2m_clear", "p2m_fetch_maps", "p2m_fetch_membership", "p2m_get_stats",
+           "p2m_paint", "p2m_paint_device", "p2m_host_alloc", "p2m_host_free", "p2m_stream_begin",
+           "p2m_stream_push", "p2m_stream_finish", "p2m_clear", "p2m_fetch_maps", "p2m_fetch_membership", "p2m_get_stats",
            "p2m_set_tile", "p2m_products", "p2m_write_outputs", "p2m_make_maptable", "p2m_selftest_math",
            "p2m_selftest_copy", "p2m_version"]
 
@@ -45,7 +46,8 @@ class Stats(C.Structure):
                 ("cut_low", C.c_double), ("cut_high", C.c_double), ("ms_total", C.c_double),
                 ("ms_decode", C.c_double), ("ms_bin", C.c_double), ("ms_paint", C.c_double),
                 ("ms_finish", C.c_double), ("tile_w", C.c_int32), ("tile_wg", C.c_int32),
-                ("ms_tile", C.c_double)]
+                ("ms_tile", C.c_double), ("h2d_bytes", C.c_int64), ("ms_h2d", C.c_double),
+                ("h2d_pinned", C.c_int32), ("n_redo_tiles", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -76,6 +78,11 @@ def load():
     L.p2m_load_table.argtypes = [vp, C.c_int, P(i32), P(C.c_float), P(C.c_float), i64]
     L.p2m_paint.argtypes = [vp, P(C.c_float), i64, i64]
     L.p2m_paint_device.argtypes = [vp, vp, i64, i64]
+    L.p2m_host_alloc.argtypes = [vp, i64, P(vp)]
+    L.p2m_host_free.argtypes = [vp, vp]
+    L.p2m_stream_begin.argtypes = [vp, i64]
+    L.p2m_stream_push.argtypes = [vp, P(C.c_float), i64, i64]
+    L.p2m_stream_finish.argtypes = [vp]
     L.p2m_clear.argtypes = [vp]
     L.p2m_fetch_maps.argtypes = [vp, C.c_int, P(C.c_float), P(d), P(C.c_float), P(d)]
     L.p2m_fetch_membership.argtypes = [vp, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32), P(C.c_uint64)]
@@ -119,6 +126,9 @@ class Painter:
 
     def close(self):
         if self.ctx:
+            for addr in list(getattr(self, "_pinned", {})):
+                self.L.p2m_host_free(self.ctx, C.c_void_p(addr))
+            self._pinned = {}
             self.L.p2m_destroy(self.ctx)
             self.ctx = C.c_void_p()
 
@@ -161,6 +171,37 @@ class Painter:
 
     def paint_device(self, dev_ptr, n, index_base=0):
         _chk(self.L.p2m_paint_device(self.ctx, C.c_void_p(dev_ptr), n, index_base), "p2m_paint_device")
+
+    def stream_begin(self, n_total):
+        _chk(self.L.p2m_stream_begin(self.ctx, int(n_total)), "p2m_stream_begin")
+
+    def stream_push(self, halos, index_base=0):
+        """Push one block; the block's buffer must stay untouched until the
+        next push (or the finish) returns."""
+        h = halos if isinstance(halos, np.ndarray) else np.asarray(halos)
+        if h.dtype != np.float32 or not h.flags.c_contiguous:
+            raise ValueError("stream_push needs a C-contiguous float32 (n, 7) array (it is read asynchronously)")
+        _chk(self.L.p2m_stream_push(self.ctx, _p(h, C.c_float), h.size // 7, index_base), "p2m_stream_push")
+
+    def stream_finish(self):
+        _chk(self.L.p2m_stream_finish(self.ctx), "p2m_stream_finish")
+
+    def host_array(self, n_halos):
+        """An (n_halos, 7) float32 array in page-locked host memory owned by
+        this context (freed with free_host_array or when the painter closes)."""
+        ptr = C.c_void_p()
+        nbytes = max(1, int(n_halos)) * 28
+        _chk(self.L.p2m_host_alloc(self.ctx, nbytes, C.byref(ptr)), "p2m_host_alloc")
+        buf = (C.c_float * (nbytes // 4)).from_address(ptr.value)
+        arr = np.frombuffer(buf, np.float32, count=int(n_halos) * 7).reshape(-1, 7)
+        self._pinned = getattr(self, "_pinned", {})
+        self._pinned[ptr.value] = buf
+        return arr
+
+    def free_host_array(self, arr):
+        addr = arr.ctypes.data
+        if getattr(self, "_pinned", {}).pop(addr, None) is not None:
+            _chk(self.L.p2m_host_free(self.ctx, C.c_void_p(addr)), "p2m_host_free")
 
     def clear(self):
         _chk(self.L.p2m_clear(self.ctx), "p2m_clear")

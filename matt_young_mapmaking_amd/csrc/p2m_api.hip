@@ -44,9 +44,10 @@ const double CKMS = 2.99792458e10 / 1e5;   /* cosmology.f90:23 */
 
 struct Ctr {                 /* layout of one shard row of the device counter block; OVF is
                                 the paint counter (PAINT + OVF, shard 0 only) of binning
-                                overflow-list entries */
-    enum { DEC = 1, ITEMS = 4, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, OVF = 7, TOTAL = P2M_CTR_STRIDE,
-           ALL = P2M_CTR_STRIDE * P2M_SHARDS };
+                                overflow-list entries; ERR counts failed device-side range
+                                checks, LOST the tiles marked for the repaint pass */
+    enum { ERR = 0, DEC = 1, LOST = 2, ITEMS = 4, N_FLAT = 5, N_CAND = 6, N_CAND0 = 7, PAINT = 8, OVF = 7,
+           TOTAL = P2M_CTR_STRIDE, ALL = P2M_CTR_STRIDE * P2M_SHARDS };
 };
 
 }  // namespace
@@ -54,7 +55,7 @@ struct Ctr {                 /* layout of one shard row of the device counter bl
 struct p2m_ctx {
     p2m_config cfg;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[10];
+    hipEvent_t ev[10] = {};
     hipStream_t fs_stream = nullptr;            /* the flat-sky painter, beside the binning */
     hipEvent_t ev_dec = nullptr, ev_fs = nullptr;
     unsigned long long *h_pin = nullptr;   /* pinned host words for mid-call readbacks */
@@ -77,6 +78,7 @@ struct p2m_ctx {
     uint4 *d_fb = nullptr;
     int64_t fb_cap = 0;
     float fr_clamp = 0.f, u_k1 = 0.f, u_k0 = 0.f, u_max = 0.f;
+    int32_t ir_clamp = 1;
     int64_t hpnpix = 0, fsnpix = 0;
     double *d_hp64[P2M_MAXMAPS] = {nullptr};
     float *d_hp32[P2M_MAXMAPS] = {nullptr};
@@ -118,8 +120,28 @@ struct p2m_ctx {
     size_t scan_tmp_fs_bytes = 0;
     uint2 *d_segs = nullptr;
     int64_t seg_cap = 0;
-    float *d_stage = nullptr;
-    int64_t stage_cap = 0;
+    /* host catalogues: two device slots that pieces of a block are copied
+     * into on the copy stream, decoded from on the painter's stream */
+    hipStream_t h2d_stream = nullptr;
+    hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    hipEvent_t ev_push = nullptr, ev_h2d_end = nullptr;
+    float *d_slot[2] = {nullptr, nullptr};
+    int64_t slot_cap = 0;                       /* halos per slot */
+    /* the streamed pass in progress (p2m_stream_*; p2m_paint* run one) */
+    bool in_stream = false;
+    int64_t stream_total = 0, stream_pushed = 0, stream_h2d_bytes = 0;
+    int n_host_push = 0, n_pieces = 0;
+    bool stream_pinned = true;
+    /* per tile: 1 if the tile's deferred pixels did not all fit in the list
+     * (the tile then wrote nothing and is painted again) */
+    uint8_t *d_lost = nullptr;
+    int n_redo = 0;
+    /* the FP32 painter's last launch, kept for the repaint pass */
+    TileArgs last_t;
+    FallbackArgs last_f;
+    int last_grid = 0;
+    int64_t last_nlisted = 0;
+    bool last_fast = false;
     uint32_t *d_cand = nullptr;
     int64_t cand_cap = 0;
     /* painted halos in spatial order (tiled painters) */
@@ -184,10 +206,11 @@ static int build_tiles(p2m_ctx *c, int log2w)
         for (int64_t t = c->tile_base[r - 1]; t < e; ++t) tile_ring[(size_t)t] = r;
     }
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_off); dfree(c->d_tile_ring);
-    dfree(c->d_tile_desc);
+    dfree(c->d_tile_desc); dfree(c->d_lost);
     c->d_tile_base = nullptr; c->d_tile_cnt = nullptr; c->d_tile_off = nullptr;
-    c->d_tile_ring = nullptr; c->d_tile_desc = nullptr;
+    c->d_tile_ring = nullptr; c->d_tile_desc = nullptr; c->d_lost = nullptr;
     int rc = 0;
+    rc |= dmalloc((void **)&c->d_lost, (size_t)((acc + 3) / 4) * 4);
     rc |= dmalloc((void **)&c->d_tile_ring, sizeof(int32_t) * acc);
     rc |= dmalloc((void **)&c->d_tile_desc, sizeof(TileDesc) * acc);
     rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
@@ -197,6 +220,7 @@ static int build_tiles(p2m_ctx *c, int log2w)
     HIPCHK(hipMemcpy(c->d_tile_base, c->tile_base.data(), sizeof(int32_t) * c->n_rings, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_tile_ring, tile_ring.data(), sizeof(int32_t) * acc, hipMemcpyHostToDevice));
     p2m_launch_tile_desc(ns, log2w, acc, c->d_tile_ring, c->d_tile_base, c->d_tile_desc, c->stream);
+    p2m_launch_fill_u32((uint32_t *)c->d_lost, (acc + 3) / 4, 0u, c->stream);
     HIPCHK(hipGetLastError());
     if (!c->d_ring_z) {
         if (dmalloc((void **)&c->d_ring_z, sizeof(double) * c->n_rings)) return P2M_E_NOMEM;
@@ -225,6 +249,7 @@ const char *p2m_strerror(int code)
     case P2M_E_STATE: return "call out of order (run parameters or table missing)";
     case P2M_E_IO: return "file could not be written";
     case P2M_E_NODEVICE: return "no GPU visible";
+    case P2M_E_INTERNAL: return g_hip_detail[0] ? g_hip_detail : "device-side consistency check failed";
     default: return "unknown error";
     }
 }
@@ -255,13 +280,17 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
     memset(&c->st, 0, sizeof(c->st));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return P2M_E_HIP; }
     if (hipStreamCreateWithFlags(&c->fs_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->h2d_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_dec, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fs, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_fs, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_copy[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_copy[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_free[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_free[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_push, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&c->ev_h2d_end) != hipSuccess) {
         (void)hipGetLastError();
-        if (c->fs_stream) (void)hipStreamDestroy(c->fs_stream);
-        if (c->ev_dec) (void)hipEventDestroy(c->ev_dec);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
+        p2m_destroy(c);
         return P2M_E_HIP;
     }
     for (int i = 0; i < 10; ++i) (void)hipEventCreate(&c->ev[i]);
@@ -303,6 +332,15 @@ void p2m_destroy(p2m_ctx *c)
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fs_stream) (void)hipStreamSynchronize(c->fs_stream);
+    if (c->h2d_stream) (void)hipStreamSynchronize(c->h2d_stream);
+    dfree(c->d_slot[0]); dfree(c->d_slot[1]); dfree(c->d_lost);
+    for (int i = 0; i < 2; ++i) {
+        if (c->ev_copy[i]) (void)hipEventDestroy(c->ev_copy[i]);
+        if (c->ev_free[i]) (void)hipEventDestroy(c->ev_free[i]);
+    }
+    if (c->ev_push) (void)hipEventDestroy(c->ev_push);
+    if (c->ev_h2d_end) (void)hipEventDestroy(c->ev_h2d_end);
+    if (c->h2d_stream) (void)hipStreamDestroy(c->h2d_stream);
     for (int m = 0; m < P2M_MAXMAPS; ++m) {
         dfree(c->d_hp64[m]); dfree(c->d_hp32[m]); dfree(c->d_fs64[m]); dfree(c->d_flat[m]); dfree(c->d_prof[m]);
     }
@@ -312,9 +350,9 @@ void p2m_destroy(p2m_ctx *c)
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_pos); dfree(c->d_ovf); dfree(c->d_ctr);
     dfree(c->d_tile_base); dfree(c->d_tile_cnt); dfree(c->d_tile_off); dfree(c->d_scan_tmp); dfree(c->d_scan_tmp_fs);
     dfree(c->d_tile_ring); dfree(c->d_tile_desc); dfree(c->d_group); dfree(c->d_ring_z); dfree(c->d_fs_cnt); dfree(c->d_fs_off);
-    dfree(c->d_segs); dfree(c->d_stage); dfree(c->d_cand);
+    dfree(c->d_segs); dfree(c->d_cand);
     dfree(c->d_halo_s); dfree(c->d_hot); dfree(c->d_bin); dfree(c->d_keys); dfree(c->d_vals); dfree(c->d_sort_tmp);
-    for (int i = 0; i < 10; ++i) (void)hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 10; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->ev_dec) (void)hipEventDestroy(c->ev_dec);
     if (c->ev_fs) (void)hipEventDestroy(c->ev_fs);
@@ -326,6 +364,7 @@ void p2m_destroy(p2m_ctx *c)
 int p2m_clear(p2m_ctx *c)
 {
     if (!c) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     HIPCHK(hipSetDevice(c->cfg.device));
     /* the tiled painters store every HEALPix pixel on the first paint after a
      * clear, so only the plain painter (atomics) needs a zeroed map */
@@ -365,6 +404,7 @@ static int ensure_hp_valid(p2m_ctx *c)
 int p2m_set_run(p2m_ctx *c, const p2m_run *r)
 {
     if (!c || !r) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (!(r->h > 0) || !(r->omegam > 0)) return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
     c->run = *r;
@@ -420,6 +460,7 @@ int p2m_zofr(p2m_ctx *c, double chi, double *z)
 int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr_f[6], const float *table, int64_t n_table)
 {
     if (!c || !hdr_i || !hdr_f || !table) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
     if (hdr_i[0] < 2 || hdr_i[1] < 2 || hdr_i[2] < 2) return P2M_E_SIZE;
     int64_t nt = 2LL * hdr_i[0] * hdr_i[1] * hdr_i[2];
@@ -433,6 +474,16 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     HIPCHK(hipSetDevice(c->cfg.device));
     memcpy(c->hdr_i, hdr_i, sizeof(c->hdr_i));
     memcpy(c->hdr_f, hdr_f, sizeof(c->hdr_f));
+    if (c->cfg.precision != P2M_PREC_PLAIN_FP64 && c->cfg.nside > 0) {
+        /* the tiled painters stage the radial nodes in LDS: the default tile
+         * shape must fit with this table; a shape set earlier that no longer
+         * fits goes back to the library's choice */
+        const int ml = c->cfg.precision == P2M_PREC_FP64 ? 1 : 2;
+        const int mem = c->d_count != nullptr;
+        if (p2m_tile_lds_bytes(ml, c->cfg.n_maps, 10, mem, hdr_i[2]) > 160 * 1024) return P2M_E_SIZE;
+        if (c->tuned && c->log2w && p2m_tile_lds_bytes(ml, c->cfg.n_maps, c->log2w, mem, hdr_i[2]) > 160 * 1024)
+            c->tuned = false;
+    }
     TabGeom &g = c->g;
     g.nchit = hdr_i[0]; g.nmht = hdr_i[1]; g.nrt = hdr_i[2];
     g.chimint = hdr_f[0]; g.chimaxt = hdr_f[1]; g.mhmint = hdr_f[2]; g.mhmaxt = hdr_f[3];
@@ -479,7 +530,13 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     c->u_k1 = (float)(0.5 * 0.6931471805599453 / g.drt);   /* log2(r^2) -> (ln r)/drt */
     c->u_k0 = (float)(g.lr0 / g.drt);
     c->u_max = (float)(g.nrt - 1);
-    c->fr_clamp = (float)(p2m_log(g.rmint + 1e-5) - g.lr0);
+    {
+        /* r < rmint is clamped to rmint + 1e-5 (maptable.f90:190); its cell
+         * and fraction as interpolate_table forms them (:196, :200), in FP64 */
+        const double lrc = p2m_log(g.rmint + 1e-5);
+        c->ir_clamp = (int)((lrc - g.lr0) / g.drt) + 1;
+        c->fr_clamp = (float)(lrc - (g.lr0 + (double)(c->ir_clamp - 1) * g.drt));
+    }
     c->have_table[slot] = true;
     return 0;
 }
@@ -487,6 +544,7 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
 int p2m_set_tile(p2m_ctx *c, int32_t tile_w, int32_t tile_wg)
 {
     if (!c) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (tile_w == 0 && tile_wg == 0) { c->tuned = false; return 0; }
     if (!is_pow2(tile_w) || tile_w < 64 || tile_w > (1 << p2m_tile_max_log2w())) return P2M_E_INVALID;
     if (tile_wg < 64 || tile_wg > 512 || tile_wg % 64) return P2M_E_INVALID;
@@ -611,6 +669,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
 {
     const int mode = c->cfg.precision;
     unsigned long long *pc = c->d_ctr + Ctr::PAINT;
+    c->last_fast = false;
     PaintArgs pa;
     memset(&pa, 0, sizeof(pa));
     pa.halos = c->d_halo; pa.n_halo = n_halo; pa.nside = c->cfg.nside; pa.npix_fs = c->cfg.npix_fs;
@@ -698,6 +757,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         }
         b.n_items = n_items; b.runs = c->d_runs; b.pos = c->d_pos; b.group_halo = c->d_group;
         b.ovf = c->d_ovf; b.ovf_cap = c->ovf_cap; b.n_ovf = pc + Ctr::OVF;
+        b.n_tiles = c->n_tiles; b.err = c->d_ctr + Ctr::ERR;
         p2m_launch_fill_u32(c->d_tile_cnt, c->n_tiles + 1, 0u, c->stream);
         p2m_launch_seg_count(b, c->stream);
         { int rc = scan_counts(c, c->d_tile_cnt, c->d_tile_off, c->n_tiles + 1); if (rc) return rc; }
@@ -735,9 +795,16 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             for (int sh = 0; sh < P2M_SHARDS; ++sh) nlisted += hb[sh * Ctr::TOTAL + Ctr::PAINT + 0];
         }
         if (mode == P2M_PREC_FP32_COMP) {
-            /* deferred FP64 pixels: ~5e-4 of the listed pixels in practice */
-            int64_t need = (int64_t)(nlisted / 16) + (1 << 20);
-            if (need > c->fb_cap) {
+            /* deferred FP64 pixels: ~5e-4 of the listed pixels in practice;
+             * a list that fills up costs a repaint of the tiles that lost
+             * pixels, after the list has grown (P2M_FB_CAP caps it, so that
+             * the repaint can be exercised) */
+            int64_t need = (int64_t)(nlisted / 256) + (1 << 18);
+            if (const char *e = getenv("P2M_FB_CAP")) {
+                const long long cap = atoll(e);
+                if (cap >= P2M_SHARDS && cap < need) need = cap;
+            }
+            if (need != c->fb_cap && (need > c->fb_cap || getenv("P2M_FB_CAP"))) {
                 dfree(c->d_fb); c->d_fb = nullptr; c->fb_cap = 0;
                 if (dmalloc((void **)&c->d_fb, sizeof(uint4) * need)) return P2M_E_NOMEM;
                 c->fb_cap = need;
@@ -759,11 +826,12 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         t.n_maps = c->cfg.n_maps; t.profile = c->cfg.profile; t.accumulate = c->hp_zero ? 0 : 1;
         t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.desc = c->d_tile_desc; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
         t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
-        t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp;
+        t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp; t.ir_clamp = c->ir_clamp;
         t.band_c = 4e-6f;
         t.inv_node2f = c->d_inv_node2f;
         t.n_prof = c->n_prof; t.tab_q = c->d_profq;
         t.fb = c->d_fb; t.fb_count = pc + 6; t.fb_cap = c->fb_cap;
+        t.lost = c->d_lost; t.redo = 0; t.n_halo = n_halo; t.err = c->d_ctr + Ctr::ERR; t.n_lost = c->d_ctr + Ctr::LOST;
         for (int m = 0; m < c->cfg.n_maps; ++m) {
             t.tab_flat[m] = c->d_flat[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
         }
@@ -790,8 +858,12 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             f.g = c->g; f.n_flat = c->n_flat;
             for (int m = 0; m < c->cfg.n_maps; ++m) { f.tab_flat[m] = c->d_flat[m]; f.map32[m] = c->d_hp32[m]; }
             f.count = c->d_count; f.hash = c->d_hash; f.counters = pc;
+            f.lost = c->d_lost; f.tile_base = c->d_tile_base; f.log2w = c->log2w; f.n_halo = n_halo;
+            f.err = c->d_ctr + Ctr::ERR;
             if (p2m_launch_fallback(f, (int64_t)(nlisted / 256) + 1024, c->stream)) return P2M_E_HIP;
+            c->last_t = t; c->last_f = f; c->last_grid = grid; c->last_nlisted = (int64_t)nlisted;
         }
+        c->last_fast = mode == P2M_PREC_FP32_COMP;
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
         HIPCHK(hipStreamWaitEvent(c->stream, c->ev_fs, 0));
     } else {
@@ -833,11 +905,67 @@ static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat, hipStream_t s)
     return 0;
 }
 
+/* The FP32 painter's deferred-pixel list filled up in some sub-list: grow
+ * it so that every sub-list holds the whole demand, and paint the tiles
+ * marked lost again (they wrote nothing; entries they did store are skipped
+ * by the fallback), then decide their deferred pixels.  Repeats until no
+ * tile is lost; `raw` holds the counters on entry and on return. */
+static int repaint_lost(p2m_ctx *c, unsigned long long h[Ctr::TOTAL], unsigned long long raw[Ctr::ALL])
+{
+    unsigned long long *pc = c->d_ctr + Ctr::PAINT;
+    c->n_redo = 0;
+    for (int iter = 0;; ++iter) {
+        unsigned long long demand = 0;
+        bool over = false;
+        for (int sh = 0; sh < P2M_SHARDS; ++sh) {
+            const unsigned long long v = raw[sh * Ctr::TOTAL + Ctr::PAINT + 6];
+            demand += v;
+            if ((int64_t)v > c->fb_cap / P2M_SHARDS) over = true;
+        }
+        if (!over) return 0;
+        c->n_redo += (int)h[Ctr::LOST];
+        if (iter >= 3) {
+            /* cannot happen: the grown list holds every pass's whole demand */
+            p2m_launch_fill_u32((uint32_t *)c->d_lost, (c->n_tiles + 3) / 4, 0u, c->stream);
+            (void)hipStreamSynchronize(c->stream);
+            snprintf(g_hip_detail, sizeof(g_hip_detail), "deferred-pixel list still full after %d repaints", iter);
+            return P2M_E_NOMEM;
+        }
+        const int64_t cap = (int64_t)P2M_SHARDS * (int64_t)(demand + 64);
+        dfree(c->d_fb); c->d_fb = nullptr; c->fb_cap = 0;
+        if (dmalloc((void **)&c->d_fb, sizeof(uint4) * cap)) return P2M_E_NOMEM;
+        c->fb_cap = cap;
+        p2m_launch_clear_ctr(pc, 6, 1, c->stream);
+        p2m_launch_clear_ctr(c->d_ctr, Ctr::LOST, 1, c->stream);
+        TileArgs t = c->last_t;
+        t.fb = c->d_fb; t.fb_cap = cap; t.redo = 1;
+        if (p2m_launch_tile_paint(t, P2M_PREC_FP32_COMP, c->cfg.n_maps, c->last_grid, c->wg, c->stream))
+            return P2M_E_INVALID;
+        HIPCHK(hipGetLastError());
+        FallbackArgs f = c->last_f;
+        f.fb = c->d_fb; f.fb_cap = cap;
+        if (p2m_launch_fallback(f, (int64_t)demand + 1024, c->stream)) return P2M_E_HIP;
+        int rc = read_counters(c, h, raw);
+        if (rc) return rc;
+    }
+}
+
 static int finish_stats(p2m_ctx *c)
 {
     unsigned long long h[Ctr::TOTAL], raw[Ctr::ALL];
     int rc = read_counters(c, h, raw);
     if (rc) return rc;
+    c->n_redo = 0;
+    if (c->last_fast) {
+        rc = repaint_lost(c, h, raw);
+        if (rc) return rc;
+    }
+    if (h[Ctr::ERR]) {
+        snprintf(g_hip_detail, sizeof(g_hip_detail),
+                 "%llu device-side range checks failed (segment, tile slot or deferred entry out of range)",
+                 h[Ctr::ERR]);
+        return P2M_E_INTERNAL;
+    }
     p2m_stats &s = c->st;
     s.n_cut = (int64_t)h[Ctr::DEC + 0];
     s.n_radius_err = (int64_t)h[Ctr::DEC + 2];
@@ -848,16 +976,7 @@ static int finish_stats(p2m_ctx *c)
     s.n_ring_gaps = (int64_t)h[Ctr::PAINT + 3];
     s.n_segments = (int64_t)h[Ctr::PAINT + 4];
     s.n_fallback = (int64_t)h[Ctr::PAINT + 5];
-    if (c->cfg.precision == P2M_PREC_FP32_COMP) {
-        for (int sh = 0; sh < P2M_SHARDS; ++sh) {
-            const unsigned long long v = raw[sh * Ctr::TOTAL + Ctr::PAINT + 6];
-            if ((int64_t)v > c->fb_cap / P2M_SHARDS) {
-                snprintf(g_hip_detail, sizeof(g_hip_detail), "deferred-pixel list overflow (%llu > %lld in sub-list %d)",
-                         v, (long long)(c->fb_cap / P2M_SHARDS), sh);
-                return P2M_E_NOMEM;
-            }
-        }
-    }
+    s.n_redo_tiles = c->n_redo;
     s.cut_low = c->cut_low; s.cut_high = c->cut_high;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); s.ms_total = ms;
@@ -880,51 +999,161 @@ static int finish_stats(p2m_ctx *c)
     return 0;
 }
 
-static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n, int64_t index_base, bool allow_tune)
+/* page-locked host memory (hipHostMalloc or otherwise pinned)? */
+static bool host_is_pinned(const void *p)
 {
-    if (!c || (!src && n > 0) || n < 0) return P2M_E_INVALID;
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+    return at.type == hipMemoryTypeHost;
+}
+
+/* A catalogue pass is begin -> push (one or more blocks) -> finish.  begin
+ * sizes the per-halo buffers for n_total halos and starts the call's clock;
+ * push decodes a block, copying host blocks to the device in pieces on the
+ * copy stream (two device slots used in turn, each piece decoded on the
+ * painter's stream as soon as it has arrived); finish bins and paints. */
+static int pass_begin(p2m_ctx *c, int64_t n_total)
+{
+    if (!c || n_total < 0) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (!c->have_run) return P2M_E_STATE;
     for (int m = 0; m < c->cfg.n_maps; ++m) if (!c->have_table[m]) return P2M_E_STATE;
     HIPCHK(hipSetDevice(c->cfg.device));
-    int rc = ensure_halo_cap(c, n);
+    int rc = ensure_halo_cap(c, n_total);
     if (rc) return rc;
-    p2m_launch_fill_u64(c->d_ctr, Ctr::ALL, c->stream);
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    const int64_t chunk = c->cfg.chunk_halos;
-    if (n > 0 && n > c->cand_cap) {
+    if (n_total > 0 && n_total > c->cand_cap) {
         dfree(c->d_cand); c->d_cand = nullptr; c->cand_cap = 0;
-        if (dmalloc((void **)&c->d_cand, sizeof(uint32_t) * n)) return P2M_E_NOMEM;
-        c->cand_cap = n;
+        if (dmalloc((void **)&c->d_cand, sizeof(uint32_t) * n_total)) return P2M_E_NOMEM;
+        c->cand_cap = n_total;
     }
-    if (!on_device && n > 0) {
-        int64_t need = std::min(chunk, n) * 7;
-        if (need > c->stage_cap) {
-            dfree(c->d_stage); c->d_stage = nullptr; c->stage_cap = 0;
-            if (dmalloc((void **)&c->d_stage, sizeof(float) * need)) return P2M_E_NOMEM;
-            c->stage_cap = need;
+    p2m_launch_fill_u64(c->d_ctr, Ctr::ALL, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    c->in_stream = true;
+    c->stream_total = n_total;
+    c->stream_pushed = 0;
+    c->stream_h2d_bytes = 0;
+    c->n_host_push = 0;
+    c->n_pieces = 0;
+    c->stream_pinned = true;
+    return 0;
+}
+
+static int pass_push_impl(p2m_ctx *c, const float *src, bool on_device, int64_t n, int64_t index_base)
+{
+    if (n == 0) return 0;
+    if (!on_device) {
+        /* the previous host block's copies have finished: its buffer is the
+         * caller's again (the double-buffer contract of p2m_stream_push) */
+        if (c->n_host_push > 0) HIPCHK(hipEventSynchronize(c->ev_push));
+        const bool pinned = host_is_pinned(src);
+        c->stream_pinned = c->stream_pinned && pinned;
+        /* pieces of 2^21 records (58.7 MB): long enough for the DMA engine's
+         * full rate, short enough that the decode trails the copy closely */
+        const int64_t piece = std::min<int64_t>(std::min<int64_t>(c->cfg.chunk_halos, 1 << 21), n);
+        if (piece > c->slot_cap || !c->d_slot[0] || !c->d_slot[1]) {
+            /* slots in use by an earlier piece are released only after the
+             * painter's stream has passed them */
+            HIPCHK(hipStreamSynchronize(c->stream));
+            dfree(c->d_slot[0]); dfree(c->d_slot[1]);
+            c->d_slot[0] = c->d_slot[1] = nullptr; c->slot_cap = 0;
+            const int64_t cap = std::max<int64_t>(piece, 1 << 16);
+            if (dmalloc((void **)&c->d_slot[0], sizeof(float) * 7 * cap)) return P2M_E_NOMEM;
+            if (dmalloc((void **)&c->d_slot[1], sizeof(float) * 7 * cap)) return P2M_E_NOMEM;
+            c->slot_cap = cap;
+            c->n_pieces = 0;
         }
+        /* copies start after the call's start mark, i.e. after every earlier
+         * use of the slots on the painter's stream */
+        if (c->n_host_push == 0) HIPCHK(hipStreamWaitEvent(c->h2d_stream, c->ev[0], 0));
+        for (int64_t c0 = 0; c0 < n; c0 += piece) {
+            const int64_t nc = std::min(piece, n - c0);
+            const int slot = c->n_pieces & 1;
+            if (c->n_pieces >= 2) HIPCHK(hipStreamWaitEvent(c->h2d_stream, c->ev_free[slot], 0));
+            HIPCHK(hipMemcpyAsync(c->d_slot[slot], src + 7 * c0, sizeof(float) * 7 * nc, hipMemcpyHostToDevice,
+                                  c->h2d_stream));
+            HIPCHK(hipEventRecord(c->ev_copy[slot], c->h2d_stream));
+            HIPCHK(hipStreamWaitEvent(c->stream, c->ev_copy[slot], 0));
+            int rc = decode_chunk(c, c->d_slot[slot], nc, index_base + c0);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(c->ev_free[slot], c->stream));
+            c->n_pieces++;
+        }
+        HIPCHK(hipEventRecord(c->ev_push, c->h2d_stream));
+        c->stream_h2d_bytes += (int64_t)sizeof(float) * 7 * n;
+        c->n_host_push++;
+        return 0;
     }
+    const int64_t chunk = c->cfg.chunk_halos;
     for (int64_t c0 = 0; c0 < n; c0 += chunk) {
-        int64_t nc = std::min(chunk, n - c0);
-        const float *d = src + 7 * c0;
-        if (!on_device) {
-            HIPCHK(hipMemcpyAsync(c->d_stage, d, sizeof(float) * 7 * nc, hipMemcpyHostToDevice, c->stream));
-            d = c->d_stage;
-        }
-        rc = decode_chunk(c, d, nc, index_base + c0);
+        const int64_t nc = std::min(chunk, n - c0);
+        int rc = decode_chunk(c, src + 7 * c0, nc, index_base + c0);
         if (rc) return rc;
     }
+    return 0;
+}
+
+static int pass_push(p2m_ctx *c, const float *src, bool on_device, int64_t n, int64_t index_base)
+{
+    if (!c || n < 0 || (!src && n > 0) || index_base < 0) return P2M_E_INVALID;
+    if (!c->in_stream) return P2M_E_STATE;
+    if (n > c->stream_total - c->stream_pushed) return P2M_E_SIZE;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc = pass_push_impl(c, src, on_device, n, index_base);
+    if (rc) {
+        /* the pass is abandoned: wait for what was queued */
+        (void)hipStreamSynchronize(c->h2d_stream);
+        (void)hipStreamSynchronize(c->stream);
+        c->in_stream = false;
+        return rc;
+    }
+    c->stream_pushed += n;
+    return 0;
+}
+
+static int pass_finish(p2m_ctx *c, bool allow_tune)
+{
+    if (!c) return P2M_E_INVALID;
+    if (!c->in_stream) return P2M_E_STATE;
+    c->in_stream = false;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    if (c->n_host_push > 0) HIPCHK(hipEventRecord(c->ev_h2d_end, c->h2d_stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     unsigned long long hc[Ctr::TOTAL];
-    rc = read_counters(c, hc);
+    int rc = read_counters(c, hc);
     if (rc) return rc;
     int64_t n_halo = (int64_t)std::min<unsigned long long>(hc[Ctr::N_CAND], (unsigned long long)c->halo_cap);
     int64_t n_flat = (int64_t)std::min<unsigned long long>(hc[Ctr::N_FLAT], (unsigned long long)c->halo_cap);
     rc = paint_halos(c, n_halo, n_flat, hc[Ctr::ITEMS], allow_tune);
-    if (rc) return rc;
+    if (rc) {
+        /* nothing queued may outlive the call */
+        (void)hipStreamSynchronize(c->fs_stream);
+        (void)hipStreamSynchronize(c->stream);
+        return rc;
+    }
     c->hp_valid = true;
-    c->st.n_in = n;
-    return finish_stats(c);
+    c->st.n_in = c->stream_pushed;
+    rc = finish_stats(c);
+    if (rc) return rc;
+    c->st.h2d_bytes = c->stream_h2d_bytes;
+    c->st.h2d_pinned = c->n_host_push > 0 && c->stream_pinned ? 1 : 0;
+    c->st.ms_h2d = 0.0;
+    if (c->n_host_push > 0) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev_h2d_end);
+        c->st.ms_h2d = ms;
+    }
+    return 0;
+}
+
+static int paint_common(p2m_ctx *c, const float *src, bool on_device, int64_t n, int64_t index_base, bool allow_tune)
+{
+    if (!c || (!src && n > 0) || n < 0) return P2M_E_INVALID;
+    int rc = pass_begin(c, n);
+    if (rc) return rc;
+    rc = pass_push(c, src, on_device, n, index_base);
+    if (rc) return rc;
+    return pass_finish(c, allow_tune);
 }
 
 extern "C" {
@@ -939,6 +1168,39 @@ int p2m_paint_device(p2m_ctx *c, const float *d_halos, int64_t n, int64_t index_
     return paint_common(c, d_halos, true, n, index_base, true);
 }
 
+int p2m_stream_begin(p2m_ctx *c, int64_t n_total) { return pass_begin(c, n_total); }
+
+int p2m_stream_push(p2m_ctx *c, const float *halos, int64_t n, int64_t index_base)
+{
+    return pass_push(c, halos, false, n, index_base);
+}
+
+int p2m_stream_finish(p2m_ctx *c) { return pass_finish(c, true); }
+
+int p2m_host_alloc(p2m_ctx *c, int64_t bytes, void **ptr)
+{
+    if (!c || !ptr || bytes <= 0) return P2M_E_INVALID;
+    *ptr = nullptr;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    if (hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return P2M_E_NOMEM;
+    }
+    return 0;
+}
+
+int p2m_host_free(p2m_ctx *c, void *ptr)
+{
+    if (!c) return P2M_E_INVALID;
+    if (!ptr) return 0;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    /* a block may still be read by a queued copy */
+    HIPCHK(hipStreamSynchronize(c->h2d_stream));
+    HIPCHK(hipHostFree(ptr));
+    return 0;
+}
+
 int p2m_get_stats(p2m_ctx *c, p2m_stats *st)
 {
     if (!c || !st) return P2M_E_INVALID;
@@ -949,6 +1211,7 @@ int p2m_get_stats(p2m_ctx *c, p2m_stats *st)
 int p2m_fetch_maps(p2m_ctx *c, int slot, float *hp32, double *hp64, float *fs32, double *fs64)
 {
     if (!c) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
     { int rc = ensure_hp_valid(c); if (rc) return rc; }
@@ -981,6 +1244,7 @@ int p2m_fetch_maps(p2m_ctx *c, int slot, float *hp32, double *hp64, float *fs32,
 int p2m_fetch_membership(p2m_ctx *c, uint32_t *hc, uint64_t *hh, uint32_t *fc, uint64_t *fh)
 {
     if (!c) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (!(c->cfg.flags & P2M_FLAG_MEMBERSHIP)) return P2M_E_STATE;
     HIPCHK(hipSetDevice(c->cfg.device));
     { int rc = ensure_hp_valid(c); if (rc) return rc; }
@@ -1044,9 +1308,13 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long lo
         (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[4]);
         if (ms < best_ms) { best_ms = ms; best_l = cd[0]; best_wg = cd[1]; }
         rc = p2m_clear(c); if (rc) return rc;
+        /* a trial's tiles that lost deferred pixels are not repainted: the
+         * maps are cleared anyway, and so are the marks */
+        p2m_launch_fill_u32((uint32_t *)c->d_lost, (c->n_tiles + 3) / 4, 0u, c->stream);
     }
     /* counters were polluted by the trials: reset the paint counters */
     p2m_launch_clear_ctr(c->d_ctr, Ctr::PAINT, Ctr::TOTAL - Ctr::PAINT, c->stream);
+    p2m_launch_clear_ctr(c->d_ctr, Ctr::LOST, 1, c->stream);
     int rc = build_tiles(c, best_l); if (rc) return rc;
     c->wg = best_wg; c->tuned = true; c->hp_zero = true;
     return 0;
@@ -1058,6 +1326,7 @@ extern "C" {
 int p2m_products(p2m_ctx *c, int slot, double moments[6], double rms[2], float dpdy[10000])
 {
     if (!c) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
     HIPCHK(hipSetDevice(c->cfg.device));
     { int rc = ensure_hp_valid(c); if (rc) return rc; }

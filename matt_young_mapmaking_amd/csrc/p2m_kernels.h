@@ -79,7 +79,9 @@ struct BinArgs {
     uint4 *ovf;                   /* pieces after an item's first: (tile, slot, halo, lo<<16 | len) */
     unsigned long long *n_ovf;    /* entries of ovf (one counter, not sharded) */
     int64_t ovf_cap;
+    int64_t n_tiles;
     unsigned long long *counters; /* [0] listed, [3] ring gaps, [4] segments */
+    unsigned long long *err;      /* sharded: pieces whose slot lies outside their tile's range */
 };
 
 /* Per-tile geometry, computed once per tile shape: the pixel-centre z and
@@ -111,6 +113,7 @@ struct TileArgs {
     const double *inv_node2;      /* exp(-2 (lr0 + k drt)), k = 0..nrt */
     const float *inv_node2f;      /* the same in float */
     float u_k1, u_k0, u_max, band, fr_clamp;
+    int32_t ir_clamp;             /* cell and fraction of r = rmint + 1e-5 (maptable.f90:190, 196, 200) */
     float band_c;                 /* relative chord^2 band decided in FP64 */
     double *map64[P2M_MAXMAPS];
     float *map32[P2M_MAXMAPS];
@@ -119,7 +122,14 @@ struct TileArgs {
     uint4 *fb;                    /* deferred FP64 pixels: (ring, ip, halo, 0), P2M_SHARDS sub-lists */
     unsigned long long *fb_count; /* sharded: sub-list k's length at fb_count[k * P2M_CTR_STRIDE] */
     int64_t fb_cap;               /* total; each sub-list holds fb_cap / P2M_SHARDS */
+    uint8_t *lost;                /* per tile: set when its deferred pixels did not all fit in fb;
+                                     such a tile writes nothing (it is painted again) */
+    int32_t redo;                 /* paint only the tiles marked lost, clearing the mark of each
+                                     that now fits */
+    int64_t n_halo;               /* records in hot / halos */
     unsigned long long *counters; /* [1] hp updates, [5] FP64 fallbacks */
+    unsigned long long *err;      /* sharded: segments whose halo or pixel range is out of bounds */
+    unsigned long long *n_lost;   /* sharded: tiles marked lost */
 };
 
 struct FallbackArgs {
@@ -135,7 +145,12 @@ struct FallbackArgs {
     float *map32[P2M_MAXMAPS];
     uint32_t *count;
     unsigned long long *hash;
+    const uint8_t *lost;          /* per tile: entries of a lost tile are skipped (it is painted again) */
+    const int32_t *tile_base;     /* first tile of ring r+1 */
+    int32_t log2w;
+    int64_t n_halo;
     unsigned long long *counters; /* [1] hp updates */
+    unsigned long long *err;      /* sharded: entries whose halo is out of range */
 };
 
 void p2m_launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);

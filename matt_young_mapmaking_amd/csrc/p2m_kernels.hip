@@ -728,7 +728,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
 __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
 {
     __shared__ unsigned long long lds16[16];
-    unsigned long long n_gaps = 0;
+    unsigned long long n_gaps = 0, n_err = 0;
     const int lane = __lane_id();
     const unsigned long long nunit = (B.n_items + 64 * EMIT_U - 1) / (64 * EMIT_U);
     unsigned long long ubeg, uend, ustep;
@@ -778,18 +778,23 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
         }
         int32_t tile[EMIT_U][2], plo[EMIT_U][2], len[EMIT_U][2];
         bool live[EMIT_U];
-        unsigned long long to[EMIT_U];
+        unsigned long long to[EMIT_U], tn[EMIT_U];
 #pragma unroll
         for (int u = 0; u < EMIT_U; ++u) {
             const int32_t np = (int32_t)(run[u] >> 16);
             live[u] = item[u] < B.n_items && np != 0;
             if (live[u]) first_two_pieces((int32_t)(run[u] & 0xffffu), np, npr[u], tb[u], B.log2w, tile[u], plo[u], len[u]);
-            to[u] = live[u] ? B.tile_off[tile[u][0]] : 0;
+            const int64_t tl = live[u] && tile[u][0] >= 0 && tile[u][0] < B.n_tiles ? tile[u][0] : 0;
+            to[u] = live[u] ? B.tile_off[tl] : 0;
+            tn[u] = live[u] ? B.tile_off[tl + 1] : 0;
         }
+        /* a slot outside its tile's range is an error, never a store */
 #pragma unroll
-        for (int u = 0; u < EMIT_U; ++u)
-            if (live[u])
-                B.segs[to[u] + ps[u]] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][0] << 16) | (uint32_t)len[u][0]);
+        for (int u = 0; u < EMIT_U; ++u) {
+            if (!live[u]) continue;
+            if (tile[u][0] < 0 || tile[u][0] >= B.n_tiles || to[u] + ps[u] >= tn[u]) { n_err++; continue; }
+            B.segs[to[u] + ps[u]] = make_uint2((uint32_t)hi[u], ((uint32_t)plo[u][0] << 16) | (uint32_t)len[u][0]);
+        }
         /* an empty ring between non-empty rings of one disc, counted once per
          * gap (the row misalignment of discedge2fulldisc, quirk Q15) */
 #pragma unroll
@@ -805,9 +810,11 @@ __global__ void __launch_bounds__(256) k_seg_emit(BinArgs B)
     for (unsigned long long o = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; o < nov;
          o += (unsigned long long)gridDim.x * blockDim.x) {
         const uint4 v = B.ovf[o];
+        if (v.x >= (uint64_t)B.n_tiles || B.tile_off[v.x] + v.y >= B.tile_off[v.x + 1]) { n_err++; continue; }
         B.segs[B.tile_off[v.x] + v.y] = make_uint2(v.z, v.w);
     }
     block_count(&B.counters[3], n_gaps, lds16);
+    block_count(B.err, n_err, lds16);
 }
 
 /* ------------------------------------------------------------------ */
@@ -945,7 +952,7 @@ __device__ static inline void lds_barrier()
 }
 
 #define FSEG 256                   /* segments staged per batch */
-#define FB_LDS 32                  /* deferred pixels buffered in LDS (flushed half full) */
+#define FB_LDS 32                  /* deferred pixels of a tile buffered in LDS (flushed per tile) */
 #define PMAX 3072                  /* pixels per batch (>= the widest tile, 2048) */
 
 /* the part of a batch both painters share: segment starts and the pixel
@@ -1117,7 +1124,7 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
     ExactSeg *sl = (ExactSeg *)(smem_raw + ((acc_bytes + mem_bytes + 15) & ~(size_t)15));
     __shared__ unsigned long long lds16[16];
     const int tid = threadIdx.x;
-    unsigned long long n_upd = 0;
+    unsigned long long n_upd = 0, n_err = 0;
     __amdgpu_buffer_rsrc_t trs[NM];
     for (int m = 0; m < NM; ++m)
         trs[m] = __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_flat[m], (short)0, (int)(A.n_flat * 4), 0x00020000);
@@ -1137,9 +1144,13 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
             uint32_t len = 0;
             if (tid < nstage) {
                 const uint2 sg = A.segs[b0 + tid];
-                const HaloRec &h = A.halos[sg.x];
-                sl->lo[tid] = sg.y >> 16;
-                len = sg.y & 0xffffu;
+                /* a segment must name a record and lie inside the tile */
+                const uint32_t slo = sg.y >> 16, slen = sg.y & 0xffffu;
+                const bool ok = sg.x < (uint64_t)A.n_halo && slo + slen <= (uint32_t)G.width;
+                if (!ok) n_err++;
+                const HaloRec &h = A.halos[ok ? sg.x : 0u];
+                sl->lo[tid] = slo;
+                len = ok ? slen : 0u;
                 sl->hx[tid] = h.hx; sl->hy[tid] = h.hy; sl->hz[tid] = h.hz; sl->dth[tid] = h.dtheta;
                 sl->vh[tid] = h.vh; sl->chir[tid] = h.chi_r; sl->fc[tid] = h.fc; sl->fm[tid] = h.fm;
                 sl->imh[tid] = h.imh; sl->ichi[tid] = h.ichi; sl->cout[tid] = h.chi_out;
@@ -1191,6 +1202,7 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
         lds_barrier();
     }
     block_count(&A.counters[1], n_upd, lds16);
+    block_count(A.err, n_err, lds16);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1219,6 +1231,7 @@ struct FastSeg {
     float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];   /* corner weights times the velocity factor */
     int32_t colbase[FSEG];
     uint32_t nfb;
+    uint32_t lost;                 /* a deferred pixel of this tile found the list full */
 };
 
 /* sin^2(x) for |x| <= pi/2: the odd Taylor polynomial of sin to x^15 (the
@@ -1295,12 +1308,12 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
         __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_q, (short)0, (int)(A.n_prof * NM * 4), 0x00020000);
     const int so_m = nrt * NM * 4, so_c = nrt * A.g.nmht * NM * 4, so_mc = so_m + so_c;
     const int tid = threadIdx.x;
-    unsigned long long n_upd = 0, n_fb = 0;
+    unsigned long long n_upd = 0, n_fb = 0, n_err = 0;
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
     unsigned long long *const fbc = A.fb_count + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE;
     uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
     for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
-    if (tid == 0) sl->nfb = 0;
+    if (tid == 0) { sl->nfb = 0; sl->lost = 0; }
     lds_barrier();
 
     int64_t tbeg, tend, tstep;
@@ -1315,7 +1328,12 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
         const TileDesc G = Gn;
         const uint64_t sbeg = sbn, send = sen;
         bool pending = t + tstep < tend;
+        if (A.redo && !A.lost[t]) {      /* repaint pass: only the tiles that lost pixels */
+            if (pending) { Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1]; }
+            continue;
+        }
         const double kphi = G.kphi;
+        unsigned long long tu = 0, tf = 0;   /* this tile's updates and deferrals */
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
@@ -1331,10 +1349,15 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
             }
             if (tid < nstage) {
-                const TileHalo &h = A.hot[sg.x];
+                /* a segment must name a record and lie inside the tile; one
+                 * that does not is counted as an error and painted empty */
+                const uint32_t slo = sg.y >> 16, slen = sg.y & 0xffffu;
+                const bool ok = sg.x < (uint64_t)A.n_halo && slo + slen <= (uint32_t)G.width;
+                if (!ok) n_err++;
+                const TileHalo &h = A.hot[ok ? sg.x : 0u];
                 sl->halo[tid] = sg.x;
-                sl->lo[tid] = sg.y >> 16;
-                len = sg.y & 0xffffu;
+                sl->lo[tid] = slo;
+                len = ok ? slen : 0u;
                 const double dz = G.zr - h.zz0, ds = G.sth - h.s0;
                 sl->A[tid] = (float)(dz * dz + ds * ds);
                 sl->B[tid] = (float)((4.0 * G.sth) * h.s0);
@@ -1387,20 +1410,23 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                     else {
                         const unsigned long long gs = atomicAdd(fbc, 1ULL);
                         if (gs < (unsigned long long)fbs) fbr[gs] = e;
+                        else sl->lost = 1u;
                     }
-                    n_fb++;
+                    tf++;
                     continue;
                 }
                 const bool clampr = u < 0.0f;                     /* r < rmint: r = rmint + 1e-5 */
-                /* floor(u) + 1 clamped to [1, nrt-1] (u < 0 gives 1) */
-                const int ir = __builtin_amdgcn_fmed3f(fl + 1.0f, 1.0f, (float)(nrt - 1));
+                /* floor(u) + 1 clamped to [1, nrt-1]; a clamped r (r = rmint + 1e-5)
+                 * has the cell the host computed for it */
+                const int ir_u = __builtin_amdgcn_fmed3f(fl + 1.0f, 1.0f, (float)(nrt - 1));
+                const int ir = clampr ? A.ir_clamp : ir_u;
                 /* both sides evaluated, then selected: no branch in the loop */
                 const float fr_in = half_log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 const float fr = clampr ? A.fr_clamp : fr_in;
                 /* the 2 NM floats of each corner column: every map at nodes ir-1, ir */
                 const int vo = (sl->colbase[s] + (ir - 1)) * (4 * NM);
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
-                n_upd++;
+                tu++;
                 float ta[2 * NM], tb[2 * NM], tc[2 * NM], td[2 * NM];
                 load_floats<2 * NM>(trs, vo, 0, ta);
                 load_floats<2 * NM>(trs, vo, so_m, tb);
@@ -1428,11 +1454,28 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
             Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
         }
         lds_barrier();
+        /* the tile's deferred pixels still in LDS join the list before the
+         * tile is written: a tile whose deferred pixels do not all fit in the
+         * list writes nothing and is marked for the repaint pass */
+        const uint32_t nfl = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
+        if (nfl) {
+            if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nfl);
+            lds_barrier();
+            for (uint32_t k = tid; k < nfl; k += blockDim.x)
+                if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
+        }
+        const bool lost = sl->lost != 0u || (nfl && fb_base + nfl > (unsigned long long)fbs);
+        if (tid == 0) {
+            if (lost) { A.lost[t] = 1; atomicAdd(A.n_lost + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE, 1ULL); }
+            else if (A.redo) A.lost[t] = 0;
+        }
+        if (!lost) { n_upd += tu; n_fb += tf; }
         const int64_t pix0 = G.first + G.ip0;
         /* the branch on accumulate is taken outside the loops: a select
          * inside lets the compiler load the map speculatively (a map read and
          * a wait on it per tile) even when the tile only stores */
-        if (A.accumulate) {
+        if (lost) {
+        } else if (A.accumulate) {
             for (int q = tid; q < G.width; q += blockDim.x) {
                 for (int m = 0; m < NM; ++m) {
                     float *dst = A.map32[m] + pix0 + q;
@@ -1452,29 +1495,14 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 }
             }
         }
-        /* flush the deferred-pixel buffer when half full */
-        if (sl->nfb >= FB_LDS / 2) {
-            const uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
-            if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
-            lds_barrier();
-            for (uint32_t k = tid; k < nf; k += blockDim.x)
-                if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
-            lds_barrier();
-            if (tid == 0) sl->nfb = 0;
-        }
         lds_barrier();
-    }
-    {
-        const uint32_t nf = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
-        if (nf) {
-            if (tid == 0) fb_base = atomicAdd(fbc, (unsigned long long)nf);
-            lds_barrier();
-            for (uint32_t k = tid; k < nf; k += blockDim.x)
-                if (fb_base + k < (unsigned long long)fbs) fbr[fb_base + k] = sl->fb[k];
-        }
+        /* every thread has read nfb and lost; the next tile's first batch
+         * starts with a barrier, which orders this reset before its deferrals */
+        if (tid == 0) { sl->nfb = 0; sl->lost = 0; }
     }
     block_count(&A.counters[1], n_upd, lds16);
     block_count(&A.counters[5], n_fb, lds16);
+    block_count(A.err, n_err, lds16);
 }
 
 /* Deferred pixels of k_tile_fast, decided and valued with the exact FP64
@@ -1488,11 +1516,15 @@ __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
     unsigned long long n = A.fb_count[blockIdx.y * P2M_CTR_STRIDE];
     if (n > fbs) n = fbs;
     const uint4 *fbr = A.fb + blockIdx.y * fbs;
-    unsigned long long n_upd = 0;
+    unsigned long long n_upd = 0, n_err = 0;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
          i += (unsigned long long)gridDim.x * blockDim.x) {
         const uint4 e = fbr[i];
         const int32_t ring = (int32_t)e.x, ip = (int32_t)e.y;
+        if (!(e.z < (uint64_t)A.n_halo && ring >= 1 && ring <= 4 * A.nside - 1 && ip >= 0)) { n_err++; continue; }
+        /* the entries of a tile that lost some of its deferred pixels: the
+         * repaint pass decides that tile again */
+        if (A.lost[A.tile_base[ring - 1] + (ip >> A.log2w)]) continue;
         /* bytes [0, 32), [80, 112) and [128, 160) of the record: the fields
          * of the exact test and value, requested before the ring geometry */
         const uint4 *hs = reinterpret_cast<const uint4 *>(A.halos + A.order[e.z]);
@@ -1530,6 +1562,7 @@ __global__ void __launch_bounds__(256) k_fallback(FallbackArgs A)
         }
     }
     block_count(&A.counters[1], n_upd, lds16);
+    block_count(A.err, n_err, lds16);
 }
 
 int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)

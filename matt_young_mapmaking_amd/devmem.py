@@ -60,3 +60,19 @@ class DeviceArray:
 def synchronize():
     if hip().hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")
+
+
+def h2d_copy_ms(dev, host_ptr, nbytes, reps=5, device=0):
+    """Median wall time (ms) of a synchronous host-to-device copy of nbytes
+    from host_ptr (pinned or pageable) into the device buffer `dev`."""
+    import time
+    h = hip()
+    if h.hipSetDevice(device) != 0:
+        raise RuntimeError("hipSetDevice failed")
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        if h.hipMemcpy(C.c_void_p(dev), C.c_void_p(host_ptr), nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy H2D failed")
+        ts.append((time.perf_counter() - t) * 1e3)
+    return float(np.median(ts))

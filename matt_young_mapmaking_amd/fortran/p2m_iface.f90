@@ -40,6 +40,9 @@ module p2m_iface
      real(c_double)     :: ms_total, ms_decode, ms_bin, ms_paint, ms_finish
      integer(c_int32_t) :: tile_w, tile_wg
      real(c_double)     :: ms_tile
+     integer(c_int64_t) :: h2d_bytes
+     real(c_double)     :: ms_h2d
+     integer(c_int32_t) :: h2d_pinned, n_redo_tiles
   end type p2m_stats
 
   interface
@@ -88,6 +91,36 @@ module p2m_iface
        real(c_float), intent(in) :: halos(*)
        integer(c_int64_t), value :: n, index_base
      end function p2m_paint
+
+     integer(c_int) function p2m_host_alloc(ctx, bytes, ptr) bind(C, name="p2m_host_alloc")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: bytes
+       type(c_ptr), intent(out) :: ptr
+     end function p2m_host_alloc
+
+     integer(c_int) function p2m_host_free(ctx, ptr) bind(C, name="p2m_host_free")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, ptr
+     end function p2m_host_free
+
+     integer(c_int) function p2m_stream_begin(ctx, n_total) bind(C, name="p2m_stream_begin")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n_total
+     end function p2m_stream_begin
+
+     integer(c_int) function p2m_stream_push(ctx, halos, n, index_base) bind(C, name="p2m_stream_push")
+       import :: c_int, c_ptr, c_float, c_int64_t
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: halos(*)
+       integer(c_int64_t), value :: n, index_base
+     end function p2m_stream_push
+
+     integer(c_int) function p2m_stream_finish(ctx) bind(C, name="p2m_stream_finish")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function p2m_stream_finish
 
      integer(c_int) function p2m_get_stats(ctx, st) bind(C, name="p2m_get_stats")
        import :: c_int, c_ptr, p2m_stats

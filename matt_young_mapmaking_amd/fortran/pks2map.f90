@@ -17,6 +17,12 @@
 ! catalogue pass; map k is written as <fileout>_<table basename>_{fs.map,hp.fits,dp.bin}.
 ! Inputs are validated before the GPU is touched; any failure exits with a
 ! nonzero status and writes no output file.
+!
+! The catalogue goes to the GPU in blocks of 2^22 halos read into two
+! page-locked buffers in turn (p2m_stream_push): the copy of one block runs
+! while the next is read.  The scramble and centre branches need every halo
+! on the host first, so with those the whole catalogue is read into one
+! page-locked array and painted with p2m_paint.
 program pks2map
   use iso_c_binding
   use p2m_iface
@@ -36,9 +42,11 @@ program pks2map
   integer(c_int32_t) :: nhalo_hdr
   integer(c_int32_t) :: hdr_i(3), hdr_i0(3)
   real(c_float) :: rthmax, hdr_f(6), hdr_f0(6)
-  real(c_float), allocatable :: halos(:,:)
+  real(c_float), pointer :: halos(:,:)
   real(c_float), allocatable :: tabs(:,:)
   integer(c_int64_t) :: nh, ntv, fsize
+  integer :: ucat
+  logical :: whole
   real(c_double) :: moments(6), rms(2), chi_lo, chi_hi
   real(c_float) :: dpdy(10000)
   integer(c_int64_t) :: hpnpix
@@ -75,20 +83,14 @@ program pks2map
 13 format('Healpix Nside:      ', i0, /, 'Number of pixels:   ', i0)
   if (hpnpix < 0) call fail('nside must be a power of 2', 2)
 
-  ! ---------------------------------------------------------------- catalogue
-  open(newunit=u, file=trim(filein), access='stream', form='unformatted', status='old', action='read', iostat=ios)
+  ! ---------------------------------------------------------------- catalogue header (pksc.f90:46-48)
+  open(newunit=ucat, file=trim(filein), access='stream', form='unformatted', status='old', action='read', iostat=ios)
   if (ios /= 0) call fail('cannot open catalogue '//trim(filein), 3)
-  inquire(unit=u, size=fsize)
-  read(u, iostat=ios) nhalo_hdr, rthmax
+  inquire(unit=ucat, size=fsize)
+  read(ucat, iostat=ios) nhalo_hdr, rthmax
   if (ios /= 0 .or. nhalo_hdr < 0) call fail('bad catalogue header in '//trim(filein), 3)
   if (fsize /= 8_8 + 28_8 * int(nhalo_hdr, 8)) call fail('catalogue size does not match its header (truncated?)', 3)
   nh = nhalo_hdr
-  allocate(halos(7, max(nh, 1_8)))
-  if (nh > 0) then
-     read(u, pos=9, iostat=ios) halos(:, 1:nh)
-     if (ios /= 0) call fail('catalogue read failed', 3)
-  end if
-  close(u)
 
   ! ---------------------------------------------------------------- tables
   ntab = 1
@@ -159,6 +161,19 @@ program pks2map
   write(*, 14) chi_lo / 1e3, chi_hi / 1e3
 14 format('cut_low, cut_high:  ', f4.2, 1x, f4.2, ' Gpc')
 
+  ! the whole catalogue on the host: for the scramble and centre branches,
+  ! and for the optional dump of the records as painted
+  call get_environment_variable('P2M_DUMP_CATALOGUE', env)
+  whole = scramble == 1 .or. centre /= 0 .or. len_trim(env) > 0
+  if (whole) then
+     call pinned_block(max(nh, 1_8), halos)
+     if (nh > 0) then
+        read(ucat, pos=9, iostat=ios) halos(:, 1:nh)
+        if (ios /= 0) call fail('catalogue read failed', 3)
+     end if
+     close(ucat)
+  end if
+
   if (scramble == 1 .or. centre /= 0) then
      call host_branches(halos, nh, real(chi_lo), real(chi_hi), chihview, scramble, centre)
      run%flags = P2M_RUN_PRECUT
@@ -166,7 +181,6 @@ program pks2map
   end if
 
   ! optional: the records as painted (after cut, scramble, centring), in .pksc layout
-  call get_environment_variable('P2M_DUMP_CATALOGUE', env)
   if (len_trim(env) > 0) then
      open(newunit=u, file=trim(env), access='stream', form='unformatted', status='replace', action='write', &
           iostat=ios)
@@ -204,7 +218,11 @@ program pks2map
   end do
 
   ! ---------------------------------------------------------------- halo loop on the GPU
-  call check(p2m_paint(ctx, halos, nh, 0_c_int64_t), 'p2m_paint')
+  if (whole) then
+     call check(p2m_paint(ctx, halos, nh, 0_c_int64_t), 'p2m_paint')
+  else
+     call stream_catalogue()
+  end if
   call check(p2m_get_stats(ctx, st), 'p2m_get_stats')
   if (st%n_radius_err > 0) write(*, '(a,i0,a)') 'Warning: ', st%n_radius_err, &
        ' halos with disc radius outside [0,pi] skipped'
@@ -233,6 +251,49 @@ program pks2map
   call p2m_destroy(ctx)
 
 contains
+
+  ! an (7, n) array in page-locked host memory of the context
+  subroutine pinned_block(n, a)
+    integer(c_int64_t), intent(in) :: n
+    real(c_float), pointer, intent(out) :: a(:,:)
+    type(c_ptr) :: p
+    call check(p2m_host_alloc(ctx, 28_c_int64_t * n, p), 'p2m_host_alloc')
+    call c_f_pointer(p, a, [7_c_int64_t, n])
+  end subroutine pinned_block
+
+  ! The block read of loadpksc (pksc.f90:60-66) for one rank holding the
+  ! whole catalogue, block by block into two page-locked buffers in turn:
+  ! a buffer is refilled only after the next push has returned, when its
+  ! copy to the device is complete.
+  subroutine stream_catalogue()
+    integer(c_int64_t), parameter :: blk = 4194304_c_int64_t
+    real(c_float), pointer :: b1(:,:), b2(:,:), bk(:,:)
+    integer(c_int64_t) :: i0, nb, nblk
+    integer :: k
+    nblk = min(max(nh, 1_c_int64_t), blk)
+    call pinned_block(nblk, b1)
+    call pinned_block(nblk, b2)
+    call check(p2m_stream_begin(ctx, nh), 'p2m_stream_begin')
+    i0 = 0
+    k = 0
+    do while (i0 < nh)
+       nb = min(blk, nh - i0)
+       if (mod(k, 2) == 0) then
+          bk => b1
+       else
+          bk => b2
+       end if
+       read(ucat, pos=9_c_int64_t + 28_c_int64_t * i0, iostat=ios) bk(:, 1:nb)
+       if (ios /= 0) call fail('catalogue read failed', 3)
+       call check(p2m_stream_push(ctx, bk, nb, i0), 'p2m_stream_push')
+       i0 = i0 + nb
+       k = k + 1
+    end do
+    close(ucat)
+    call check(p2m_stream_finish(ctx), 'p2m_stream_finish')
+    call check(p2m_host_free(ctx, c_loc(b1)), 'p2m_host_free')
+    call check(p2m_host_free(ctx, c_loc(b2)), 'p2m_host_free')
+  end subroutine stream_catalogue
 
   subroutine fail(msg, code)
     character(len=*), intent(in) :: msg

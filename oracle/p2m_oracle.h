@@ -11,6 +11,9 @@
  *           values; elementary functions from p2m_math.h (portable, so the
  *           GPU can reproduce pixel membership bit for bit).  This is the
  *           grading oracle.
+ *   ref32 : the reference's own float32 arithmetic (p2m_oracle_ref32.c), to
+ *           state how far maps in the reference's precision lie from the
+ *           FP64 oracle, and as the CPU baseline.
  * The library is built twice: libp2mo.so (portable math) and libp2mo_libm.so
  * (same source, glibc libm), to show the portable math does not change any
  * membership decision on the test catalogues.
@@ -117,6 +120,21 @@ double p2mo_ranf(int32_t seed[4]);
  * compacts only positions and Rth; quirk Q18). */
 void p2mo_host_branches(float *h, int64_t *n, float cut_low, float cut_high, float chihview,
                         int scramble, int centre);
+
+/* ref32 mode (p2m_oracle_ref32.c): the reference's float32 arithmetic with
+ * its mixed-mode promotions, on contiguous halo blocks per thread with
+ * thread-private float maps summed in thread order (hpmap/fsmap are
+ * overwritten; counts/hashes accumulate).  setup (6 doubles per input halo,
+ * may be NULL) receives (z, mh, rh, chi, dtheta, 1) of each painted halo. */
+int p2mo_paint_ref32(const p2mo_run *run, const int32_t hdr_i[3], const float hdr_f[6],
+                     const float *padded, int64_t n_padded, const float *halos, int64_t n,
+                     float *hpmap, float *fsmap, uint32_t *hp_count, uint64_t *hp_hash,
+                     uint32_t *fs_count, uint64_t *fs_hash, double *setup, p2mo_stats *st);
+/* the float32 distance tables of set_rofztable / set_zofrtable (10000 each) */
+int p2mo_ref32_tables(double omegam, double omegal, double h, float *rofz, float *zofr);
+/* interpolate_table in float32 (mh clamped in place) */
+float p2mo_ref32_interpolate(const int32_t hdr_i[3], const float hdr_f[6], const float *padded, int64_t n_padded,
+                             double omegam, double omegal, double h, float theta, float z, float *mh, int profile);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,14 @@ def lib(flavour="portable"):
     L.p2mo_host_branches.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int64), C.c_float, C.c_float, C.c_float,
                                      C.c_int, C.c_int]
     L.p2mo_host_branches.restype = None
+    L.p2mo_paint_ref32.argtypes = [C.POINTER(Run), C.POINTER(i32), C.POINTER(C.c_float), C.POINTER(C.c_float), i64,
+                                   C.POINTER(C.c_float), i64, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint64), C.POINTER(d), C.POINTER(Stats)]
+    L.p2mo_ref32_tables.argtypes = [d, d, d, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.p2mo_ref32_interpolate.argtypes = [C.POINTER(i32), C.POINTER(C.c_float), C.POINTER(C.c_float), i64, d, d, d,
+                                         C.c_float, C.c_float, C.POINTER(C.c_float), C.c_int]
+    L.p2mo_ref32_interpolate.restype = C.c_float
     _LIBS[name] = L
     return L
 
@@ -160,6 +168,38 @@ class Oracle:
             raise RuntimeError(f"p2mo_paint failed: {rc}")
         out.update(hp=hp, fs=fs, hp_count=hc, hp_hash=hh, fs_count=fc, fs_hash=fh, stats=st.as_dict())
         return out
+
+    def paint_ref32(self, halos, nside=0, npix=0, fov_deg=10.0, zmin=0.05, zmax=1.2, chihview=0.0, model=1,
+                    profile=1, mmin=1e13, nthreads=1, maps=True, membership=False, setup=False):
+        """The halo loop in the reference's float32 arithmetic (ref32 mode):
+        float32 maps summed from thread-private maps, optional membership and
+        per-halo setup values (z, mh, rh, chi, dtheta, painted)."""
+        halos = np.ascontiguousarray(halos, dtype=np.float32).reshape(-1, 7)
+        run = Run(nside, npix, fov_deg, zmin, zmax, chihview, model, profile, mmin, nthreads)
+        hpn = 12 * nside * nside if nside > 0 else 0
+        hp = np.zeros(hpn, np.float32) if (maps and hpn) else None
+        fs = np.zeros(npix * npix, np.float32) if (maps and npix) else None
+        hc = np.zeros(hpn, np.uint32) if (membership and hpn) else None
+        hh = np.zeros(hpn, np.uint64) if (membership and hpn) else None
+        fc = np.zeros(npix * npix, np.uint32) if (membership and npix) else None
+        fh = np.zeros(npix * npix, np.uint64) if (membership and npix) else None
+        su = np.zeros((halos.shape[0], 6), np.float64) if setup else None
+        st = Stats()
+        rc = self.L.p2mo_paint_ref32(C.byref(run), _ptr(self.hdr_i, C.c_int32), _ptr(self.hdr_f, C.c_float),
+                                     _ptr(self.table_padded, C.c_float), self.table_padded.size,
+                                     _ptr(halos, C.c_float), halos.shape[0], _ptr(hp, C.c_float), _ptr(fs, C.c_float),
+                                     _ptr(hc, C.c_uint32), _ptr(hh, C.c_uint64), _ptr(fc, C.c_uint32),
+                                     _ptr(fh, C.c_uint64), _ptr(su, C.c_double), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"p2mo_paint_ref32 failed: {rc}")
+        return dict(hp=hp, fs=fs, hp_count=hc, hp_hash=hh, fs_count=fc, fs_hash=fh, setup=su, stats=st.as_dict())
+
+    def interpolate_ref32(self, theta, z, mh, profile=1):
+        m = C.c_float(mh)
+        v = self.L.p2mo_ref32_interpolate(_ptr(self.hdr_i, C.c_int32), _ptr(self.hdr_f, C.c_float),
+                                          _ptr(self.table_padded, C.c_float), self.table_padded.size,
+                                          0.25, 0.75, 0.7, C.c_float(theta), C.c_float(z), C.byref(m), profile)
+        return float(v), float(m.value)
 
 
 def pad_table(table):
@@ -228,6 +268,14 @@ def host_branches(halos, cut_low, cut_high, chihview=0.0, scramble=0, centre=0):
     lib().p2mo_host_branches(_ptr(h, C.c_float), C.byref(n), C.c_float(cut_low), C.c_float(cut_high),
                              C.c_float(chihview), int(scramble), int(centre))
     return h[:n.value].copy()
+
+
+def ref32_tables(omegam=0.25, omegal=0.75, h=0.7):
+    """The float32 rofz and zofr tables of cosmology.f90:51-105."""
+    rofz = np.zeros(10000, np.float32)
+    zofr = np.zeros(10000, np.float32)
+    lib().p2mo_ref32_tables(omegam, omegal, h, _ptr(rofz, C.c_float), _ptr(zofr, C.c_float))
+    return rofz, zofr
 
 
 def ranf(seed):

@@ -1,5 +1,6 @@
 """GPU parity through the C ABI against the oracle on the same seeded
-catalogues (sizes the oracle finishes in seconds).
+catalogues, up to the full C2 (2e6 halos, nside 2048) and C3 (2e7 halos,
+nside 4096) workloads, which the oracle paints in seconds on the host cores.
 
 Bar: pixel membership bit-exact (per-pixel count and halo-hash sum, HEALPix
 and flat sky), no pixel zero in one map and not the other, FP64 paths within
@@ -24,6 +25,8 @@ CASES = {
     "C2s": ("C2", 200000, 2048, 1024),
     "C3s": ("C3", 200000, 4096, 1024),
     "lowres": ("C2", 30000, 64, 64),       # sub-pixel discs, empty middle rings
+    "C2": ("C2", 2_000_000, 2048, 1024),   # the full workloads
+    "C3": ("C3", 20_000_000, 4096, 1024),
 }
 TOL = {abi.PREC_PLAIN_FP64: (1e-12, 1e-13), abi.PREC_FP64: (1e-12, 1e-13), abi.PREC_FP32_COMP: (1e-6, 1e-7)}
 
@@ -38,6 +41,7 @@ _REF = {}
 
 def reference(case, table):
     if case not in _REF:
+        _REF.clear()       # one case's oracle maps at a time (4 GB at C3)
         cfg, n, nside, npix = CASES[case]
         h = catalog.config_catalog(cfg, n=n)
         o = p2mo.Oracle(*table)
@@ -87,6 +91,9 @@ def test_parity(case, prec, table):
         assert abs(hp.sum() - ref["hp"].sum()) / abs(ref["hp"].sum()) <= ytol
     if prec == abi.PREC_FP32_COMP:
         assert st["n_fallback"] < 0.01 * max(1, st["n_upd_hp"])
+        assert st["n_redo_tiles"] == 0
+    print(f"[gpu parity] case={case} precision={prec}: max rel err {rel_err(hp, ref['hp']):.3e}, "
+          f"updates {st['n_upd_hp']}", flush=True)
 
 
 def test_precut_and_index_base(table):
@@ -103,3 +110,86 @@ def test_precut_and_index_base(table):
     assert np.array_equal(hh, ref["hp_hash"]) and np.array_equal(hc, ref["hp_count"])
     assert np.array_equal(fh, ref["fs_hash"])
     assert rel_err(hp, ref["hp"]) <= 1e-12
+
+
+@pytest.mark.parametrize("pinned", [False, True])
+def test_streamed_blocks_equal_one_call(table, pinned):
+    """The catalogue pushed block by block (p2m_stream_*; pageable or
+    page-locked host memory, two buffers used in turn) paints exactly what one
+    p2m_paint call over the whole catalogue paints: the oracle's membership,
+    counts and values."""
+    print(f"\n[gpu parity] streamed blocks, pinned={pinned}", flush=True)
+    h, ref = reference("C2s", table)
+    blk = 70001                                     # blocks that do not divide the catalogue
+    with abi.Painter(nside=2048, npix_fs=1024, precision=abi.PREC_FP32_COMP, membership=True) as p:
+        p.set_run()
+        p.load_table(0, *table)
+        bufs = [p.host_array(blk), p.host_array(blk)] if pinned else [np.empty((blk, 7), np.float32) for _ in range(2)]
+        p.stream_begin(h.shape[0])
+        for k, i0 in enumerate(range(0, h.shape[0], blk)):
+            nb = min(blk, h.shape[0] - i0)
+            b = bufs[k % 2][:nb]
+            b[:] = h[i0:i0 + nb]                    # refilled only after the next push returned
+            p.stream_push(b, index_base=i0)
+        p.stream_finish()
+        st = p.stats()
+        hp, fs = p.maps()
+        hc, hh, fc, fh = p.membership()
+    assert st["h2d_pinned"] == (1 if pinned else 0)
+    assert st["h2d_bytes"] == 28 * h.shape[0] and st["n_in"] == h.shape[0]
+    assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+    assert np.array_equal(fc, ref["fs_count"]) and np.array_equal(fh, ref["fs_hash"])
+    for k in ("n_cut", "n_painted", "n_listed_hp", "n_upd_hp", "n_upd_fs"):
+        assert st[k] == ref["stats"][k], k
+    assert rel_err(hp, ref["hp"]) <= 1e-6
+    assert rel_err(fs, ref["fs"]) <= 1e-12
+
+
+def test_stream_errors(table):
+    """Out-of-order and oversized stream calls are errors that leave the
+    context usable."""
+    h = catalog.config_catalog("C2", n=1000)
+    with abi.Painter(nside=64, npix_fs=0, precision=abi.PREC_FP32_COMP) as p:
+        p.set_run()
+        p.load_table(0, *table)
+        with pytest.raises(abi.P2MError) as e:
+            p.stream_push(h)
+        assert e.value.code == -6                    # push before begin
+        p.stream_begin(500)
+        with pytest.raises(abi.P2MError) as e:
+            p.stream_push(h)                         # more than announced
+        assert e.value.code == -2
+        with pytest.raises(abi.P2MError) as e:
+            p.clear()                                # no other call during a pass
+        assert e.value.code == -6
+        p.stream_push(h[:500].copy())
+        p.stream_finish()
+        assert p.stats()["n_in"] == 500
+        p.paint(h)                                   # usable again
+
+
+def test_deferred_list_overflow_repaints(table, monkeypatch):
+    """A deferred-pixel list far too small for the call (forced from the host
+    with P2M_FB_CAP) loses no pixel: the tiles that could not store all their
+    deferred pixels write nothing and are painted again once the list has
+    grown.  Membership and counts equal the oracle's, values stay within 1e-6,
+    and a second call on the same context (accumulating) does too."""
+    print("\n[gpu parity] deferred list overflow", flush=True)
+    h, ref = reference("C3s", table)
+    monkeypatch.setenv("P2M_FB_CAP", str(64 * 4))
+    with abi.Painter(nside=4096, npix_fs=1024, precision=abi.PREC_FP32_COMP, membership=True) as p:
+        p.set_tile(1024, 256)
+        p.set_run()
+        p.load_table(0, *table)
+        half = h.shape[0] // 2
+        p.paint(h[:half], index_base=0)
+        s1 = p.stats()
+        p.paint(h[half:], index_base=half)
+        s2 = p.stats()
+        hp, _ = p.maps()
+        hc, hh, _, _ = p.membership()
+    assert s1["n_redo_tiles"] > 0 and s2["n_redo_tiles"] > 0
+    assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+    assert s1["n_upd_hp"] + s2["n_upd_hp"] == ref["stats"]["n_upd_hp"]
+    assert int(np.count_nonzero((hp != 0) != (ref["hp"] != 0))) == 0
+    assert rel_err(hp, ref["hp"]) <= 1e-6
