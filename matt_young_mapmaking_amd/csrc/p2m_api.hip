@@ -71,14 +71,15 @@ struct p2m_ctx {
     TabGeom g;
     float *d_flat[P2M_MAXMAPS] = {nullptr};
     float *d_prof[P2M_MAXMAPS] = {nullptr};    /* the profile's slice, zero-padded (FP32 painter) */
-    float *d_profq = nullptr;                   /* the slices interleaved map by map (FP32 painter) */
-    int64_t n_flat = 0, n_prof = 0;
+    float *d_quad = nullptr;                    /* corner-quad table of all maps (FP32 painter) */
+    int64_t n_flat = 0, n_prof = 0, n_quad = 0;
     double *d_inv_node2 = nullptr;
     float *d_inv_node2f = nullptr;
     uint4 *d_fb = nullptr;
     int64_t fb_cap = 0;
     float fr_clamp = 0.f, u_k1 = 0.f, u_k0 = 0.f, u_max = 0.f;
     int32_t ir_clamp = 1;
+    float u_band = 2.5e-4f;     /* half-width of the deferral band around cell boundaries, in cells */
     int64_t hpnpix = 0, fsnpix = 0;
     double *d_hp64[P2M_MAXMAPS] = {nullptr};
     float *d_hp32[P2M_MAXMAPS] = {nullptr};
@@ -344,7 +345,7 @@ void p2m_destroy(p2m_ctx *c)
     for (int m = 0; m < P2M_MAXMAPS; ++m) {
         dfree(c->d_hp64[m]); dfree(c->d_hp32[m]); dfree(c->d_fs64[m]); dfree(c->d_flat[m]); dfree(c->d_prof[m]);
     }
-    dfree(c->d_profq);
+    dfree(c->d_quad);
     dfree(c->d_count); dfree(c->d_hash); dfree(c->d_fs_count); dfree(c->d_fs_hash);
     dfree(c->d_rofz); dfree(c->d_zofr); dfree(c->d_inv_node2); dfree(c->d_inv_node2f); dfree(c->d_fb);
     dfree(c->d_halo); dfree(c->d_flat_idx); dfree(c->d_nr); dfree(c->d_ring_off); dfree(c->d_runs); dfree(c->d_pos); dfree(c->d_ovf); dfree(c->d_ctr);
@@ -496,9 +497,9 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     /* padding covers every index interpolate_table can form past the end (Q6) */
     int64_t pad = 2LL * g.nrt * (g.nmht + 2) + 4LL * g.nrt + 64;
     /* the painters read the tables through buffer resources with 32-bit byte
-     * offsets: one padded table, and the interleaved profile tables of all
-     * slots, must each stay below 2 GiB */
-    if ((nt + pad) * 4 > INT32_MAX || (nt + pad) / 2 * 4 * c->cfg.n_maps > INT32_MAX) return P2M_E_SIZE;
+     * offsets: one padded table, and the corner-quad table of all slots (16
+     * bytes per map and node of each cell), must each stay below 2 GiB */
+    if ((nt + pad) * 4 > INT32_MAX || (nt / 2) * 16 * c->cfg.n_maps > INT32_MAX) return P2M_E_SIZE;
     c->n_flat = nt + pad;
     if (!c->d_flat[slot] && dmalloc((void **)&c->d_flat[slot], sizeof(float) * c->n_flat)) return P2M_E_NOMEM;
     std::vector<float> flat(c->n_flat, 0.f);
@@ -513,8 +514,10 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
         for (int64_t i = 0; i < nt / 2; ++i) prof[(size_t)i] = table[(p - 1) + 2 * i];
         if (!c->d_prof[slot] && dmalloc((void **)&c->d_prof[slot], sizeof(float) * c->n_prof)) return P2M_E_NOMEM;
         HIPCHK(hipMemcpy(c->d_prof[slot], prof.data(), sizeof(float) * c->n_prof, hipMemcpyHostToDevice));
-        if (!c->d_profq && dmalloc((void **)&c->d_profq, sizeof(float) * c->n_prof * c->cfg.n_maps)) return P2M_E_NOMEM;
-        p2m_launch_interleave(c->d_prof[slot], c->n_prof, c->cfg.n_maps, slot, c->d_profq, c->stream);
+        c->n_quad = (int64_t)g.nchit * g.nmht * g.nrt * 4 * c->cfg.n_maps;
+        if (!c->d_quad && dmalloc((void **)&c->d_quad, sizeof(float) * c->n_quad)) return P2M_E_NOMEM;
+        p2m_launch_build_quad(c->d_prof[slot], c->n_prof, g.nrt, g.nmht, g.nchit, c->cfg.n_maps, slot, c->d_quad,
+                              c->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
     }
@@ -530,6 +533,10 @@ int p2m_load_table(p2m_ctx *c, int slot, const int32_t hdr_i[3], const float hdr
     c->u_k1 = (float)(0.5 * 0.6931471805599453 / g.drt);   /* log2(r^2) -> (ln r)/drt */
     c->u_k0 = (float)(g.lr0 / g.drt);
     c->u_max = (float)(g.nrt - 1);
+    /* the FP32 cell coordinate carries log2's error times u_k1 (about 10 on
+     * the standard grid): a finer radial grid widens the band in proportion,
+     * and a band of half a cell or more sends every interior pixel to FP64 */
+    c->u_band = std::max(2.5e-4f, 2.5e-5f * c->u_k1);
     {
         /* r < rmint is clamped to rmint + 1e-5 (maptable.f90:190); its cell
          * and fraction as interpolate_table forms them (:196, :200), in FP64 */
@@ -826,10 +833,10 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         t.n_maps = c->cfg.n_maps; t.profile = c->cfg.profile; t.accumulate = c->hp_zero ? 0 : 1;
         t.n_tiles = c->n_tiles; t.tile_base = c->d_tile_base; t.desc = c->d_tile_desc; t.tile_off = c->d_tile_off; t.segs = c->d_segs;
         t.g = c->g; t.n_flat = c->n_flat; t.inv_node2 = c->d_inv_node2;
-        t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = 2.5e-4f; t.fr_clamp = c->fr_clamp; t.ir_clamp = c->ir_clamp;
+        t.u_k1 = c->u_k1; t.u_k0 = c->u_k0; t.u_max = c->u_max; t.band = c->u_band; t.fr_clamp = c->fr_clamp; t.ir_clamp = c->ir_clamp;
         t.band_c = 4e-6f;
         t.inv_node2f = c->d_inv_node2f;
-        t.n_prof = c->n_prof; t.tab_q = c->d_profq;
+        t.n_quad = c->n_quad; t.tab_q = c->d_quad;
         t.fb = c->d_fb; t.fb_count = pc + 6; t.fb_cap = c->fb_cap;
         t.lost = c->d_lost; t.redo = 0; t.n_halo = n_halo; t.err = c->d_ctr + Ctr::ERR; t.n_lost = c->d_ctr + Ctr::LOST;
         for (int m = 0; m < c->cfg.n_maps; ++m) {
@@ -1414,15 +1421,20 @@ static bool write_block(FILE *f, const std::vector<std::string> &cards)
 }
 
 /* HEALPix map as FITS BINTABLE (fits_s_inc.f90:787-960 via cfitsio in the
- * reference; the same HDU structure and keywords written directly). */
-static int write_fits(const char *path, const float *m, int64_t npix, int32_t nside)
+ * reference; the same HDU structure and keywords written directly).  The map
+ * streams from the device in pieces of 2^24 pixels through two page-locked
+ * buffers: the copy of piece k+1 runs while piece k is byte-swapped to
+ * big-endian and written, as the reference writes its rows piece by piece
+ * (fits_s_inc.f90:921-933).  An FP64 map is rounded to float on the device
+ * first (the reference's map is real). */
+static int write_fits_dev(p2m_ctx *c, int slot, const char *path)
 {
+    const int64_t npix = c->hpnpix;
+    const int32_t nside = c->cfg.nside;
     int64_t repeat = 1024;
     if (npix < 1024) repeat = 1;
     if (nside > 1024 * 256) repeat = nside / 256;
     int64_t nrows = npix / repeat;
-    FILE *f = fopen(path, "wb");
-    if (!f) return P2M_E_IO;
     char date[32];
     time_t t = time(nullptr);
     struct tm tmv;
@@ -1449,28 +1461,78 @@ static int write_fits(const char *path, const float *m, int64_t npix, int32_t ns
         card("TFORM1", sval(tform), "data format of field: 4-byte REAL"),
         card("NSIDE", ival(nside), "the nside of the map"),
         card("ORDERING", sval("RING"), "the nside of the map")};
-    bool ok = write_block(f, prim) && write_block(f, ext);
-    std::vector<uint32_t> buf(1 << 20);
-    for (int64_t i = 0; ok && i < npix; i += (int64_t)buf.size()) {
-        int64_t k = std::min<int64_t>((int64_t)buf.size(), npix - i);
-        for (int64_t j = 0; j < k; ++j) {
-            uint32_t u;
-            memcpy(&u, &m[i + j], 4);
-            buf[j] = __builtin_bswap32(u);
-        }
-        ok = fwrite(buf.data(), 4, (size_t)k, f) == (size_t)k;
+    const int64_t piece = std::min<int64_t>(npix, 1 << 24);
+    float *hb[2] = {nullptr, nullptr};
+    float *d32 = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    FILE *f = nullptr;
+    int rc = 0;
+    if (hipHostMalloc((void **)&hb[0], sizeof(float) * piece, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&hb[1], sizeof(float) * piece, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = P2M_E_NOMEM;
     }
-    int64_t bytes = npix * 4, pad = (2880 - bytes % 2880) % 2880;
-    std::vector<char> z(pad, 0);
-    if (ok && pad) ok = fwrite(z.data(), 1, pad, f) == (size_t)pad;
-    if (fclose(f) != 0) ok = false;
-    if (!ok) { remove(path); return P2M_E_IO; }
-    return 0;
+    if (!rc && c->d_hp64[slot] && dmalloc((void **)&d32, sizeof(float) * 2 * piece)) rc = P2M_E_NOMEM;
+    if (!rc && (hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess)) {
+        (void)hipGetLastError();
+        rc = P2M_E_HIP;
+    }
+    if (!rc) {
+        f = fopen(path, "wb");
+        if (!f) rc = P2M_E_IO;
+    }
+    bool ok = !rc && write_block(f, prim) && write_block(f, ext);
+    if (!rc && !ok) rc = P2M_E_IO;
+    const int64_t npieces = (npix + piece - 1) / piece;
+    auto enqueue = [&](int64_t k) -> int {
+        const int64_t i0 = k * piece, nk = std::min(piece, npix - i0);
+        const int bslot = (int)(k & 1);
+        const float *src = c->d_hp32[slot] + (c->d_hp32[slot] ? i0 : 0);
+        if (c->d_hp64[slot]) {
+            p2m_launch_f64_to_f32(c->d_hp64[slot] + i0, d32 + bslot * piece, nk, c->stream);
+            src = d32 + bslot * piece;
+        }
+        if (hipMemcpyAsync(hb[bslot], src, sizeof(float) * nk, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipEventRecord(ev[bslot], c->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            return P2M_E_HIP;
+        }
+        return 0;
+    };
+    if (!rc && npieces > 0) rc = enqueue(0);
+    for (int64_t k = 0; !rc && k < npieces; ++k) {
+        if (k + 1 < npieces) rc = enqueue(k + 1);
+        if (rc) break;
+        const int bslot = (int)(k & 1);
+        if (hipEventSynchronize(ev[bslot]) != hipSuccess) { (void)hipGetLastError(); rc = P2M_E_HIP; break; }
+        const int64_t nk = std::min(piece, npix - k * piece);
+        uint32_t *u = reinterpret_cast<uint32_t *>(hb[bslot]);
+        for (int64_t j = 0; j < nk; ++j) u[j] = __builtin_bswap32(u[j]);
+        if (fwrite(u, 4, (size_t)nk, f) != (size_t)nk) rc = P2M_E_IO;
+        /* the buffer is refilled by the copy enqueued two pieces later, after
+         * this write has returned */
+    }
+    if (!rc) {
+        int64_t bytes = npix * 4, pad = (2880 - bytes % 2880) % 2880;
+        std::vector<char> z(pad, 0);
+        if (pad && fwrite(z.data(), 1, pad, f) != (size_t)pad) rc = P2M_E_IO;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    if (f && fclose(f) != 0 && !rc) rc = P2M_E_IO;
+    if (rc && f) remove(path);
+    for (int i = 0; i < 2; ++i) {
+        if (hb[i]) (void)hipHostFree(hb[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    dfree(d32);
+    return rc;
 }
 
 extern "C" int p2m_write_outputs(p2m_ctx *c, int slot, const char *base, int write_fs, int write_hp, int write_dp)
 {
     if (!c || !base) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
     if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
     std::string b(base);
     int rc = 0;
@@ -1502,11 +1564,11 @@ extern "C" int p2m_write_outputs(p2m_ctx *c, int slot, const char *base, int wri
         if (!ok) { remove(p.c_str()); return P2M_E_IO; }
     }
     if (write_hp && c->hpnpix > 0) {
-        std::vector<float> m(c->hpnpix);
-        rc = p2m_fetch_maps(c, slot, m.data(), nullptr, nullptr, nullptr);
+        HIPCHK(hipSetDevice(c->cfg.device));
+        rc = ensure_hp_valid(c);
         if (rc) return rc;
         std::string p = b + "_hp.fits";
-        rc = write_fits(p.c_str(), m.data(), c->hpnpix, c->cfg.nside);
+        rc = write_fits_dev(c, slot, p.c_str());
         if (rc) return rc;
     }
     return 0;

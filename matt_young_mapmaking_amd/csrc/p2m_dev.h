@@ -252,7 +252,7 @@ P2M_DEV double angdist(double ax, double ay, double az, double bx, double by, do
 /* phi half-width of the disc on a ring at height z (discphirange_at_z,
  * pix_tools.F90:712-760); -1000 = no intersection. */
 template <typename H>
-P2M_DEV double disc_dphi(const H &h, double z)
+P2M_HD double disc_dphi(const H &h, double z)
 {
     double dphi = -1000.0;
     double b = h.cosang - z * h.zz0;
@@ -270,7 +270,7 @@ P2M_DEV double disc_dphi(const H &h, double z)
 
 /* x mod n in [0, n) (modulo of pixels_on_edge); the disc edges lie within a
  * ring's length or two of zero, so the 64-bit division is rarely needed */
-P2M_DEV int64_t mod_ring(int64_t x, int64_t n)
+P2M_HD int64_t mod_ring(int64_t x, int64_t n)
 {
     if (x >= 0 && x < n) return x;
     if (x < 0 && x >= -n) return x + n;
@@ -284,7 +284,7 @@ P2M_DEV int64_t mod_ring(int64_t x, int64_t n)
  * of pixels np (0 if none) and the first in-ring index lo (pixels lo..lo+np-1
  * modulo npr). */
 template <typename H>
-P2M_DEV int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo);
+P2M_HD int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo);
 
 template <typename H>
 P2M_DEV int32_t disc_ring_run(const H &h, int32_t nside, int32_t ring, int32_t npr, int32_t kshift, int32_t &lo)
@@ -294,7 +294,7 @@ P2M_DEV int32_t disc_ring_run(const H &h, int32_t nside, int32_t ring, int32_t n
 
 /* the same, with the ring's z (ring2z) given */
 template <typename H>
-P2M_DEV int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo)
+P2M_HD int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo)
 {
     double dphi = disc_dphi(h, z);
     if (dphi >= HPX_PI) { lo = 0; return npr; }
@@ -302,6 +302,47 @@ P2M_DEV int32_t disc_ring_run_z(const H &h, double z, int32_t npr, int32_t kshif
     double shift = (double)kshift * 0.5;
     int64_t il = (int64_t)p2m_ceil(((double)npr * (h.phi0 - dphi)) / HPX_TWOPI - shift);
     int64_t ih = (int64_t)p2m_floor(((double)npr * (h.phi0 + dphi)) / HPX_TWOPI - shift);
+    if (ih < il) return 0;
+    const int64_t l = mod_ring(il, npr);
+    const int64_t hh = mod_ring(ih, npr);
+    int64_t np = mod_ring(hh - l, npr);
+    np += 1;
+    if (np > npr) np = npr;
+    lo = (int32_t)l;
+    return (int32_t)np;
+}
+
+/* The run of disc_ring_run_z with the arc half-width dphi = acos(cosdphi)
+ * taken as 2 asin(sqrt((1 - cosdphi)/2)) in FP32 (cosdphi itself, and
+ * 1 - cosdphi, in FP64 as the reference forms them).  The ends
+ * npr (phi0 -+ dphi) / 2pi - shift are formed in FP64 with the FP32 dphi;
+ * when either lies within the bound of that dphi's error (1e-6 relative,
+ * against a few ulp of the FP32 asin and sqrt) of an integer, where ceil or
+ * floor could differ from the reference's, or the arc nearly closes the ring,
+ * the exact FP64 evaluation decides.  So the returned run is the exact one
+ * for every disc and ring. */
+template <typename H>
+P2M_HD int32_t disc_ring_run_fast(const H &h, double z, int32_t npr, int32_t kshift, int32_t &lo)
+{
+    if (h.a == 0.0) return disc_ring_run_z(h, z, npr, kshift, lo);      /* no acos on this branch */
+    const double b = h.cosang - z * h.zz0;
+    double cc = 1.0 - z * z;
+    if (cc < 1.0e-12) cc = 1.0e-12;
+    const double cosdphi = b / p2m_sqrt(h.a * cc);
+    if (cosdphi < -1.0) { lo = 0; return npr; }
+    if (!(p2m_fabs(cosdphi) <= 1.0)) return 0;
+    if (1.0 + cosdphi < 1e-5) return disc_ring_run_z(h, z, npr, kshift, lo);   /* dphi near pi */
+    const float sh = sqrtf((float)(0.5 * (1.0 - cosdphi)));
+    const double dphi = (double)(2.0f * asinf(sh));
+    const double shift = (double)kshift * 0.5;
+    const double k = (double)npr * (1.0 / HPX_TWOPI);
+    const double tl = (h.phi0 - dphi) * k - shift;
+    const double th = (h.phi0 + dphi) * k - shift;
+    const double eps = k * (1e-6 * dphi + 1e-12) + 1e-9;
+    const double fl = tl - p2m_floor(tl), fh = th - p2m_floor(th);
+    if (fl < eps || fl > 1.0 - eps || fh < eps || fh > 1.0 - eps) return disc_ring_run_z(h, z, npr, kshift, lo);
+    const int64_t il = (int64_t)p2m_ceil(tl);
+    const int64_t ih = (int64_t)p2m_floor(th);
     if (ih < il) return 0;
     const int64_t l = mod_ring(il, npr);
     const int64_t hh = mod_ring(ih, npr);

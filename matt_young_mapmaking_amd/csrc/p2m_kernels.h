@@ -105,10 +105,9 @@ struct TileArgs {
     const uint2 *segs;
     TabGeom g;
     const float *tab_flat[P2M_MAXMAPS];
-    const float *tab_q;           /* the maps' profile slices interleaved: tab_q[c * n_maps + m] is
-                                     slice m at flat index c (zero-padded), so one pixel's values of
-                                     every map at radial nodes c and c+1 are 2 n_maps adjacent floats */
-    int64_t n_prof;               /* floats in each slice */
+    const float *tab_q;           /* corner-quad table (k_build_quad): a pixel's nodes ir-1, ir of its
+                                     cell, 4 corners of every map, are 8 n_maps contiguous floats */
+    int64_t n_quad;               /* floats in tab_q */
     int64_t n_flat;
     const double *inv_node2;      /* exp(-2 (lr0 + k drt)), k = 0..nrt */
     const float *inv_node2f;      /* the same in float */
@@ -175,7 +174,8 @@ int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin,
                    hipStream_t s);
 int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
 int p2m_tile_max_log2w(void);
-void p2m_launch_interleave(const float *slice, int64_t n, int32_t n_maps, int32_t slot, float *q, hipStream_t s);
+void p2m_launch_build_quad(const float *slice, int64_t n_slice, int32_t nrt, int32_t nmht, int32_t nchit,
+                           int32_t n_maps, int32_t slot, float *quad, hipStream_t s);
 size_t p2m_tile_lds_bytes(int mode, int n_maps, int log2w, int membership, int nrt);
 int p2m_tile_occupancy(int mode, int n_maps, int wg, size_t lds);
 int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int wg, hipStream_t s);

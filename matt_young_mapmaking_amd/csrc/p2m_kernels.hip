@@ -677,7 +677,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
             int32_t npr, kshift; int64_t first;
             if (ring <= h.irmax) {
                 ring_info(B.nside, ring, npr, kshift, first);
-                np = disc_ring_run_z(h, B.ring_z[ring - 1], npr, kshift, lo);
+                np = disc_ring_run_fast(h, B.ring_z[ring - 1], npr, kshift, lo);
             }
             /* first pixel | count << 16: lo < npr and np <= npr <= 32768 for nside <= 8192 */
             B.runs[item] = (uint32_t)lo | ((uint32_t)np << 16);
@@ -1222,16 +1222,18 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
 /* --------------------------------------------------------------------- */
 
 struct FastSeg {
-    BatchLDS b;
-    double dphi0[FSEG];            /* phi offset of the ring's pixel 0 from the halo centre */
-    uint4 fb[FB_LDS];
+    float4 geo[FSEG];              /* A, B, chord^2 edge band top and bottom */
+    float4 wgt[FSEG];              /* corner weights times the velocity factor */
+    float4 aux[FSEG];              /* azimuth origin x_c, chi^2, column base (bits), lo - pre (bits) */
+    int32_t cidx[FSEG];            /* batch pixel of the azimuth origin (pixel index p - cidx = ip - ip_c) */
     uint32_t halo[FSEG];
-    uint16_t lo[FSEG];             /* first in-tile pixel (< 2048) */
-    float A[FSEG], B[FSEG], thr_hi[FSEG], chi2[FSEG];
-    float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];   /* corner weights times the velocity factor */
-    int32_t colbase[FSEG];
+    uint4 fb[FB_LDS];
+    uint32_t pre[FSEG + 1];        /* exclusive scan of the staged lengths */
+    uint32_t wsum[16];
+    uint32_t nb;
     uint32_t nfb;
     uint32_t lost;                 /* a deferred pixel of this tile found the list full */
+    uint8_t starts[4][64];         /* per wave: segment starts inside its current 64-pixel chunk */
 };
 
 /* sin^2(x) for |x| <= pi/2: the odd Taylor polynomial of sin to x^15 (the
@@ -1280,8 +1282,38 @@ __device__ static inline void load_floats(__amdgpu_buffer_rsrc_t r, int vo, int 
     for (; k < NF; ++k) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo + 4 * k, so, 0));
 }
 
+/* Scan of the staged lengths into fs->pre, and the leading segments whose
+ * pixels fit in PMAX (a segment is at most W <= PMAX pixels, so at least
+ * one).  Returns their number; P receives their pixel count.  Every thread
+ * calls; ends with a barrier. */
+__device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage, uint32_t &P)
+{
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    uint32_t v = len;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) fs->wsum[wv] = v;
+    lds_barrier();
+    uint32_t add = 0;
+    for (int k = 0; k < wv; ++k) add += fs->wsum[k];
+    v += add;
+    if (tid < nstage) fs->pre[tid + 1] = v;
+    if (tid == 0) fs->pre[0] = 0;
+    lds_barrier();
+    if (tid < nstage) {
+        const uint32_t e = fs->pre[tid + 1];
+        if (e <= PMAX && (tid + 1 == nstage || fs->pre[tid + 2] > PMAX)) fs->nb = (uint32_t)(tid + 1);
+    }
+    lds_barrier();
+    const int nb = (int)fs->nb;
+    P = fs->pre[nb];
+    return nb;
+}
+
 template <int NM>
-__global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *__restrict__ tdesc,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 1 ? 6 : 1))) k_tile_fast(TileArgs A, const TileDesc *__restrict__ tdesc,
                                                    const unsigned long long *__restrict__ toff)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -1299,21 +1331,21 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
     __shared__ unsigned long long fb_base;
     /* inner edge of the deferral band, relative to the outer one */
     const float lo_band = (float)((1.0 - (double)A.band_c) / (1.0 + (double)A.band_c));
-    const float u_zero = A.u_max + A.band, u_edge = A.u_max - A.band, half_in = 0.5f - A.band;
+    const float u_zero = A.u_max + A.band;
 
     const int nrt = A.g.nrt;
-    /* the table slices as raw buffers: a 32-bit byte offset per pixel, the
-     * corner strides as scalar offsets, reads past the slice return 0 */
+    /* the corner-quad table as a raw buffer: a 32-bit byte offset per pixel,
+     * reads past its end return 0 */
     const __amdgpu_buffer_rsrc_t trs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_q, (short)0, (int)(A.n_prof * NM * 4), 0x00020000);
-    const int so_m = nrt * NM * 4, so_c = nrt * A.g.nmht * NM * 4, so_mc = so_m + so_c;
-    const int tid = threadIdx.x;
+        __builtin_amdgcn_make_buffer_rsrc((void *)A.tab_q, (short)0, (int)(A.n_quad * 4), 0x00020000);
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6, nwv = (int)(blockDim.x >> 6);
     unsigned long long n_upd = 0, n_fb = 0, n_err = 0;
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
     unsigned long long *const fbc = A.fb_count + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE;
     uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
     for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
     if (tid == 0) { sl->nfb = 0; sl->lost = 0; }
+    uint8_t *const starts = sl->starts[wv];
     lds_barrier();
 
     int64_t tbeg, tend, tstep;
@@ -1333,6 +1365,9 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
             continue;
         }
         const double kphi = G.kphi;
+        const float hphi = (float)(0.5 * kphi);          /* x step per pixel along the ring */
+        const int32_t mr = G.ring < A.nside ? G.ring : (G.ring < 3 * A.nside ? A.nside : 4 * A.nside - G.ring);
+        const int32_t npr = 4 * mr, half_npr = npr / 2;
         unsigned long long tu = 0, tf = 0;   /* this tile's updates and deferrals */
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
@@ -1348,64 +1383,89 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 pending = false;
                 Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
             }
+            uint32_t slo = 0;
             if (tid < nstage) {
                 /* a segment must name a record and lie inside the tile; one
                  * that does not is counted as an error and painted empty */
-                const uint32_t slo = sg.y >> 16, slen = sg.y & 0xffffu;
+                slo = sg.y >> 16;
+                const uint32_t slen = sg.y & 0xffffu;
                 const bool ok = sg.x < (uint64_t)A.n_halo && slo + slen <= (uint32_t)G.width;
                 if (!ok) n_err++;
-                const TileHalo &h = A.hot[ok ? sg.x : 0u];
-                sl->halo[tid] = sg.x;
-                sl->lo[tid] = slo;
                 len = ok ? slen : 0u;
-                const double dz = G.zr - h.zz0, ds = G.sth - h.s0;
-                sl->A[tid] = (float)(dz * dz + ds * ds);
-                sl->B[tid] = (float)((4.0 * G.sth) * h.s0);
-                sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
-                sl->chi2[tid] = h.chi2;
-                const float fc = h.fc, fm = h.fm, vh = h.vh;
-                sl->w00[tid] = vh * ((1.0f - fc) * (1.0f - fm));
-                sl->w10[tid] = vh * (fc * (1.0f - fm));
-                sl->w01[tid] = vh * ((1.0f - fc) * fm);
-                sl->w11[tid] = vh * (fc * fm);
-                sl->colbase[tid] = h.colbase;
-                sl->dphi0[tid] = G.off * kphi - h.phi0;
-                if (A.count) hh[tid] = h.hash;
             }
             uint32_t P;
-            const int nb = batch_prepare(&sl->b, len, nstage, P);
+            const int nb = batch_prefix(sl, len, nstage, P);
+            if (tid < nb) {
+                const TileHalo &h = A.hot[sg.x < (uint64_t)A.n_halo ? sg.x : 0u];
+                sl->halo[tid] = sg.x;
+                const double dz = G.zr - h.zz0, ds = G.sth - h.s0;
+                const float thr_hi = (float)(h.thr * (1.0 + (double)A.band_c));
+                sl->geo[tid] = make_float4((float)(dz * dz + ds * ds), (float)((4.0 * G.sth) * h.s0), thr_hi,
+                                           thr_hi * lo_band);
+                const float fc = h.fc, fm = h.fm, vh = h.vh;
+                sl->wgt[tid] = make_float4(vh * ((1.0f - fc) * (1.0f - fm)), vh * (fc * (1.0f - fm)),
+                                           vh * ((1.0f - fc) * fm), vh * (fc * fm));
+                /* azimuth origin: the ring pixel ip_c nearest the halo centre's
+                 * azimuth, with x_c = ((ip_c + off) kphi - phi0) / 2 folded by
+                 * pi (|x_c| <= kphi / 4) in FP64; a pixel's x is then x_c plus
+                 * (ip - ip_c) steps of kphi / 2, its step count folded into
+                 * [-npr/2, npr/2] (sin^2 has period pi), so |x| <= pi/2 + kphi/4
+                 * and its FP32 error stays relative to x */
+                double ipc = __builtin_rint(h.phi0 / kphi - G.off);
+                if (ipc < 0.0) ipc += (double)npr;
+                if (ipc >= (double)npr) ipc -= (double)npr;
+                double xc = 0.5 * ((ipc + G.off) * kphi - h.phi0);
+                xc -= HPX_PI * __builtin_rint(xc / HPX_PI);
+                const int32_t qoff = (int32_t)slo - (int32_t)sl->pre[tid];
+                sl->aux[tid] = make_float4((float)xc, h.chi2, __int_as_float(h.colbase), __int_as_float(qoff));
+                sl->cidx[tid] = (int32_t)ipc - G.ip0 - qoff;
+                if (A.count) hh[tid] = h.hash;
+            }
+            lds_barrier();
             b0 += (uint64_t)nb;
-            const unsigned char *segb = (const unsigned char *)sl->b.seg_of;
-            for (uint32_t p = tid; p < P; p += blockDim.x) {
-                const int s = segb[p];
-                const int q = (int)(sl->lo[s] + (p - sl->b.pre[s]));
-                const int ip = G.ip0 + q;
-                /* azimuth offset in FP64 (the difference of two angles near
-                 * each other), folded by the period pi of sin^2 */
-                double x = 0.5 * __builtin_fma((double)ip, kphi, sl->dphi0[s]);
-                if (x > HPX_HALFPI) x -= HPX_PI;
-                const float s2 = sin2f((float)x);
-                const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
-                const float thr_hi = sl->thr_hi[s];
-                if (!(c2 < thr_hi)) continue;                      /* outside the disc */
+            /* waves take 64-pixel chunks of the batch; a lane's segment is the
+             * chunk's first segment plus the segments starting in the chunk up
+             * to its pixel (marked in the wave's LDS byte map, counted by ballot) */
+            for (uint32_t c0 = (uint32_t)wv * 64; c0 < P; c0 += (uint32_t)nwv * 64) {
+                int sa = 0, sh = nb - 1;
+                while (sa < sh) {
+                    const int mid = (sa + sh + 1) >> 1;
+                    if (sl->pre[mid] <= c0) sa = mid; else sh = mid - 1;
+                }
+                const int si = sa + 1 + lane;
+                const uint32_t st = si < nb ? sl->pre[si] : 0xffffffffu;
+                starts[lane] = 0;
+                if (st < c0 + 64) starts[st - c0] = 1;       /* st > c0: sa is the last start <= c0 */
+                const bool mine = starts[lane] != 0;
+                const unsigned long long smask = __ballot(mine);
+                const uint32_t p = c0 + (uint32_t)lane;
+                if (p >= P) continue;
+                const int s = sa + (int)__popcll(smask & ((2ULL << lane) - 1ULL));
+                const float4 aux = sl->aux[s];
+                const int q = (int)p + __float_as_int(aux.w);
+                int n = (int)p - sl->cidx[s];
+                n = n > half_npr ? n - npr : (n < -half_npr ? n + npr : n);
+                const float x = __builtin_fmaf((float)n, hphi, aux.x);
+                const float4 geo = sl->geo[s];
+                const float s2 = sin2f(x);
+                const float c2 = __builtin_fmaf(geo.y, s2, geo.x);
+                if (!(c2 < geo.z)) continue;                      /* outside the disc */
                 const float h2 = 0.25f * c2;
                 const float dd = 1.0f - 2.0f * h2;
                 const float rd = __builtin_amdgcn_rcpf(dd);
                 const float t2 = ((4.0f * h2) * (1.0f - h2)) * (rd * rd);
-                const float r2 = sl->chi2[s] * t2;
+                const float r2 = aux.y * t2;
                 const float u = __builtin_amdgcn_logf(r2) * A.u_k1 - A.u_k0;
                 const float fl = __builtin_floorf(u);
-                const float fu = u - fl;
                 const bool zero = u > u_zero;                     /* r > rmaxt: value 0 */
-                /* deferred: near the disc edge, a degenerate chord, or u near a
-                 * cell boundary (|fu - 1/2| > 1/2 - band), 0, or the r cut-off;
-                 * bitwise, so the test compiles to compares and mask logic */
-                const bool defer = (c2 > thr_hi * lo_band) | !(dd > 1e-3f) |
-                                   (!zero & ((__builtin_fabsf(u) < A.band) | (u > u_edge) |
-                                             ((u >= 0.0f) & (__builtin_fabsf(fu - 0.5f) > half_in))));
+                /* deferred: near the disc edge, a degenerate chord, or u within
+                 * the band of a cell boundary 0..u_max (u clamped to that range,
+                 * then its nearest boundary) */
+                const float ub = __builtin_amdgcn_fmed3f(u, 0.0f, A.u_max);
+                const bool defer = (c2 > geo.w) | !(dd > 1e-3f) | (__builtin_fabsf(u - __builtin_rintf(ub)) <= A.band);
                 if (defer) {
                     const uint32_t slot = atomicAdd(&sl->nfb, 1u);
-                    const uint4 e = make_uint4((uint32_t)G.ring, (uint32_t)ip, sl->halo[s], 0u);
+                    const uint4 e = make_uint4((uint32_t)G.ring, (uint32_t)(G.ip0 + q), sl->halo[s], 0u);
                     if (slot < FB_LDS) sl->fb[slot] = e;
                     else {
                         const unsigned long long gs = atomicAdd(fbc, 1ULL);
@@ -1423,26 +1483,27 @@ __global__ void __launch_bounds__(256) k_tile_fast(TileArgs A, const TileDesc *_
                 /* both sides evaluated, then selected: no branch in the loop */
                 const float fr_in = half_log1p_small_rcp(__builtin_fmaf(r2, invn[ir - 1], -1.0f));
                 const float fr = clampr ? A.fr_clamp : fr_in;
-                /* the 2 NM floats of each corner column: every map at nodes ir-1, ir */
-                const int vo = (sl->colbase[s] + (ir - 1)) * (4 * NM);
-                const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
+                /* nodes ir-1 and ir of the cell: 4 corners of every map each,
+                 * 32 NM contiguous bytes */
+                const int vo = (__float_as_int(aux.z) + (ir - 1)) * (16 * NM);
+                const float4 w = sl->wgt[s];
                 tu++;
-                float ta[2 * NM], tb[2 * NM], tc[2 * NM], td[2 * NM];
-                load_floats<2 * NM>(trs, vo, 0, ta);
-                load_floats<2 * NM>(trs, vo, so_m, tb);
-                load_floats<2 * NM>(trs, vo, so_c, tc);
-                load_floats<2 * NM>(trs, vo, so_mc, td);
+                float tq[8 * NM];
+                load_floats<8 * NM>(trs, vo, 0, tq);
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
-                    const float a0 = ta[m], a1 = ta[NM + m];
-                    const float b0v = tb[m], b1 = tb[NM + m];
-                    const float cc0 = tc[m], cc1 = tc[NM + m];
-                    const float d0 = td[m], d1 = td[NM + m];
-                    float acc4 = w00 * __builtin_fmaf(fr, a1 - a0, a0);
-                    acc4 = __builtin_fmaf(w10, __builtin_fmaf(fr, b1 - b0v, b0v), acc4);
-                    acc4 = __builtin_fmaf(w01, __builtin_fmaf(fr, cc1 - cc0, cc0), acc4);
-                    acc4 = __builtin_fmaf(w11, __builtin_fmaf(fr, d1 - d0, d0), acc4);
-                    atomicAdd(&acc[m * W + q], zero ? 0.0 : (double)acc4);
+                    const float *c0v = tq + 4 * m, *c1v = tq + 4 * (NM + m);
+                    /* the corners combined at each node, then one lerp in r */
+                    float v0 = w.x * c0v[0];
+                    v0 = __builtin_fmaf(w.y, c0v[1], v0);
+                    v0 = __builtin_fmaf(w.z, c0v[2], v0);
+                    v0 = __builtin_fmaf(w.w, c0v[3], v0);
+                    float v1 = w.x * c1v[0];
+                    v1 = __builtin_fmaf(w.y, c1v[1], v1);
+                    v1 = __builtin_fmaf(w.z, c1v[2], v1);
+                    v1 = __builtin_fmaf(w.w, c1v[3], v1);
+                    const float val = __builtin_fmaf(fr, v1 - v0, v0);
+                    atomicAdd(&acc[m * W + q], zero ? 0.0 : (double)val);
                 }
                 if (A.count) {
                     atomicAdd(&lcount[q], 1u);
@@ -1574,20 +1635,42 @@ int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-/* q[c * n_maps + slot] = slice[c]: one map's profile slice into the
- * interleaved table of the FP32 painter */
-__global__ void k_interleave(const float *slice, int64_t n, int32_t n_maps, int32_t slot, float *q)
+/* The FP32 painter's table: for every (mass, chi) cell and radial node k,
+ * the values of the four corners the pixel weights combine -- (imh, ichi),
+ * (imh+1, ichi), (imh, ichi+1), (imh+1, ichi+1) -- for every map, so that a
+ * pixel's two nodes k = ir-1, ir are 32 NM contiguous bytes:
+ *   quad[((cell * nrt + k) * n_maps + m) * 4 + corner],
+ *   cell = (imh-1) + nmht (ichi-1).
+ * Corner values come from the profile slice by the reference's flat index
+ * (maptable.f90:204-212), so the aliasing past the upper mass or chi edge
+ * (Q6) reads the next column or the zero padding as the flat table does. */
+__global__ void k_build_quad(const float *slice, int64_t n_slice, int32_t nrt, int32_t nmht, int32_t nchit,
+                             int32_t n_maps, int32_t slot, float *quad)
 {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
-        q[c * n_maps + slot] = slice[c];
+    const int64_t n = (int64_t)nrt * nmht * nchit;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k = (int32_t)(e % nrt);
+        const int64_t cell = e / nrt;
+        const int32_t imh0 = (int32_t)(cell % nmht), ichi0 = (int32_t)(cell / nmht);   /* 0-based */
+        float v[4];
+        for (int cr = 0; cr < 4; ++cr) {
+            const int64_t idx = (int64_t)k + (int64_t)nrt * ((int64_t)(imh0 + (cr & 1)) + (int64_t)nmht * (ichi0 + (cr >> 1)));
+            v[cr] = idx < n_slice ? slice[idx] : 0.0f;
+        }
+        float *q = quad + (e * n_maps + slot) * 4;
+        q[0] = v[0]; q[1] = v[1]; q[2] = v[2]; q[3] = v[3];
+    }
 }
 
-void p2m_launch_interleave(const float *slice, int64_t n, int32_t n_maps, int32_t slot, float *q, hipStream_t s)
+void p2m_launch_build_quad(const float *slice, int64_t n_slice, int32_t nrt, int32_t nmht, int32_t nchit,
+                           int32_t n_maps, int32_t slot, float *quad, hipStream_t s)
 {
+    const int64_t n = (int64_t)nrt * nmht * nchit;
     if (n <= 0) return;
     int64_t g = (n + 255) / 256;
-    if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_interleave, dim3((unsigned)g), dim3(256), 0, s, slice, n, n_maps, slot, q);
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_build_quad, dim3((unsigned)g), dim3(256), 0, s, slice, n_slice, nrt, nmht, nchit, n_maps,
+                       slot, quad);
 }
 
 /* widest tile the painters take: a segment (at most one tile wide) must fit

@@ -425,6 +425,67 @@ static inline void add_u64(uint64_t *p, uint64_t v)
     *p += v;
 }
 
+/* Per-halo part of the halo loop (pks2map.f90:144-160, 264, 284-298 and
+ * haloproject.f90:47,97-99): returns 0 if the halo fails the radial cut, 1 if
+ * it passes the cut but not the mass cut, 2 if it is painted (then the
+ * outputs are set). */
+typedef struct {
+    double redshift, mh, rh, chih, dtheta, vh;
+    double ux, uy, uz;
+} halo_setup;
+
+static int setup_one(const p2mo_run *run, const p2mo_cosmo *c, const float *h, double cut_low, double cut_high,
+                     double mypi, halo_setup *o)
+{
+    const double dmodel = bbps_delta(run->model);
+    const double delta_pksc = 200.0;                                /* pksc.f90:8-9 */
+    const double rmax = 4.0;                                        /* haloproject.f90:18 */
+    /* radial cut, pks2map.f90:144-160 (painter axes: x=pos3, y=pos2, z=pos1) */
+    double xh = (double)h[2], yh = (double)h[1], zh = (double)h[0];
+    xh = xh - run->chihview;
+    double chih = M_SQRT_((xh * xh + yh * yh) + zh * zh);
+    if (chih > cut_high || chih < cut_low) return 0;
+    /* mass, pks2map.f90:264 */
+    double rth = (double)h[6];
+    double mass = (((4.0 / 3.0) * mypi) * c->rho_0) * cube(rth);
+    /* velocity factor: profile 1 -> 1 (pksc.f90:293-294); profile 2 -> -v_r/c (intended kSZ, Q18) */
+    double vh = 1.0;
+    if (run->profile == 2) {
+        double px = h[0], py = h[1], pz = h[2];
+        double vr = (px * (double)h[3] + py * (double)h[4]) + pz * (double)h[5];
+        vr = vr / M_SQRT_((px * px + py * py) + pz * pz);
+        vh = -vr / 3.0e5;
+    }
+    /* halo loop body, pks2map.f90:284-298 */
+    xh = (double)h[2]; yh = (double)h[1]; zh = (double)h[0];
+    chih = M_SQRT_((xh * xh + yh * yh) + zh * zh);
+    double redshift = p2mo_zofr(c, chih);
+    xh = xh - run->chihview;
+    chih = M_SQRT_((xh * xh + yh * yh) + zh * zh);
+    double mh = M_SQRT_(p2mo_deltacrit(c, redshift) / dmodel) * mass;
+    if (mass < run->mmin) return 1;
+    double rh = M_POW_(((((3.0 * mh) / 4.0) / mypi) / delta_pksc) / p2mo_rhocrit(c, redshift), 1.0 / 3.0);
+    /* both painters: dtheta_halo = rmax*asin(r/chi) (haloproject.f90:47,93) */
+    o->dtheta = rmax * M_ASIN_(rh / chih);
+    o->ux = xh / chih; o->uy = yh / chih; o->uz = zh / chih;          /* haloproject.f90:97-99 */
+    o->redshift = redshift; o->mh = mh; o->rh = rh; o->chih = chih; o->vh = vh;
+    return 2;
+}
+
+int p2mo_halo_setup(const p2mo_run *run, const p2mo_cosmo *c, const float *halos, int64_t n, double *out)
+{
+    const double mypi = 4.0 * M_ATAN_(1.0);
+    const double cut_low = p2mo_rofz(c, run->zmin), cut_high = p2mo_rofz(c, run->zmax);
+    for (int64_t i = 0; i < n; ++i) {
+        halo_setup hs;
+        const int k = setup_one(run, c, halos + 7 * i, cut_low, cut_high, mypi, &hs);
+        double *o = out + 6 * i;
+        if (k == 2) { o[0] = hs.redshift; o[1] = hs.mh; o[2] = hs.rh; o[3] = hs.chih; o[4] = hs.dtheta; o[5] = 1.0; }
+        else { o[0] = o[1] = o[2] = o[3] = o[4] = 0.0; o[5] = 0.0; }
+    }
+    return 0;
+}
+
 int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
                const float *halos, int64_t n,
                double *hpmap, double *fsmap,
@@ -437,9 +498,6 @@ int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
     const double dpix = run->npix > 0 ? fov / (double)run->npix : 1.0; /* :93 */
     const double cut_low = p2mo_rofz(c, run->zmin);                 /* :134 */
     const double cut_high = p2mo_rofz(c, run->zmax);                /* :135 */
-    const double dmodel = bbps_delta(run->model);
-    const double delta_pksc = 200.0;                                /* pksc.f90:8-9 */
-    const double rmax = 4.0;                                        /* haloproject.f90:18 */
     const int32_t nside = run->nside;
     const int32_t npix = run->npix;
 
@@ -452,37 +510,15 @@ int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
         int64_t list_cap = 0;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < n; ++i) {
-            const float *h = halos + 7 * i;
-            /* radial cut, pks2map.f90:144-160 (painter axes: x=pos3, y=pos2, z=pos1) */
-            double xh = (double)h[2], yh = (double)h[1], zh = (double)h[0];
-            xh = xh - run->chihview;
-            double chih = M_SQRT_((xh * xh + yh * yh) + zh * zh);
-            if (chih > cut_high || chih < cut_low) continue;
+            halo_setup hs;
+            const int kind = setup_one(run, c, halos + 7 * i, cut_low, cut_high, mypi, &hs);
+            if (kind == 0) continue;
             n_cut++;
-            /* mass, pks2map.f90:264 */
-            double rth = (double)h[6];
-            double mass = (((4.0 / 3.0) * mypi) * c->rho_0) * cube(rth);
-            /* velocity factor: profile 1 -> 1 (pksc.f90:293-294); profile 2 -> -v_r/c (intended kSZ, Q18) */
-            double vh = 1.0;
-            if (run->profile == 2) {
-                double px = h[0], py = h[1], pz = h[2];
-                double vr = (px * (double)h[3] + py * (double)h[4]) + pz * (double)h[5];
-                vr = vr / M_SQRT_((px * px + py * py) + pz * pz);
-                vh = -vr / 3.0e5;
-            }
-            /* halo loop body, pks2map.f90:284-298 */
-            xh = (double)h[2]; yh = (double)h[1]; zh = (double)h[0];
-            chih = M_SQRT_((xh * xh + yh * yh) + zh * zh);
-            double redshift = p2mo_zofr(c, chih);
-            xh = xh - run->chihview;
-            chih = M_SQRT_((xh * xh + yh * yh) + zh * zh);
-            double mh = M_SQRT_(p2mo_deltacrit(c, redshift) / dmodel) * mass;
-            if (mass < run->mmin) continue;
+            if (kind == 1) continue;
             n_painted++;
-            double rh = M_POW_(((((3.0 * mh) / 4.0) / mypi) / delta_pksc) / p2mo_rhocrit(c, redshift), 1.0 / 3.0);
-            /* both painters: dtheta_halo = rmax*asin(r/chi) (haloproject.f90:47,93) */
-            double dtheta = rmax * M_ASIN_(rh / chih);
-            double ux = xh / chih, uy = yh / chih, uz = zh / chih;   /* haloproject.f90:97-99 */
+            const double dtheta = hs.dtheta, redshift = hs.redshift, vh = hs.vh;
+            double mh = hs.mh;
+            const double ux = hs.ux, uy = hs.uy, uz = hs.uz;
             uint64_t hh = p2mo_halo_hash(i);
 
             /* ---- projecthalo_flatsky, haloproject.f90:83-156 ---- */

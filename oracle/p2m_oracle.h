@@ -95,6 +95,11 @@ int p2mo_paint(const p2mo_run *run, const p2mo_cosmo *c, const p2mo_table *tb,
                uint32_t *fs_count, uint64_t *fs_hash,
                p2mo_stats *st);
 
+/* per-halo setup of the halo loop (pks2map.f90:144-160, 264, 284-298,
+ * haloproject.f90:47): out[6*i] = (z, mh, rh, chi, dtheta, 1) for a painted
+ * halo, zeros for a halo cut by radius or mass */
+int p2mo_halo_setup(const p2mo_run *run, const p2mo_cosmo *c, const float *halos, int64_t n, double *out);
+
 /* per-halo membership fingerprint term */
 uint64_t p2mo_halo_hash(int64_t i);
 

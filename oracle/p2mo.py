@@ -94,6 +94,7 @@ def lib(flavour="portable"):
     L.p2mo_host_branches.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int64), C.c_float, C.c_float, C.c_float,
                                      C.c_int, C.c_int]
     L.p2mo_host_branches.restype = None
+    L.p2mo_halo_setup.argtypes = [C.POINTER(Run), C.POINTER(Cosmo), C.POINTER(C.c_float), i64, C.POINTER(d)]
     L.p2mo_paint_ref32.argtypes = [C.POINTER(Run), C.POINTER(i32), C.POINTER(C.c_float), C.POINTER(C.c_float), i64,
                                    C.POINTER(C.c_float), i64, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -167,6 +168,15 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"p2mo_paint failed: {rc}")
         out.update(hp=hp, fs=fs, hp_count=hc, hp_hash=hh, fs_count=fc, fs_hash=fh, stats=st.as_dict())
+        return out
+
+    def halo_setup(self, halos, zmin=0.05, zmax=1.2, chihview=0.0, model=1, profile=1, mmin=1e13):
+        """Per-halo (z, mh, rh, chi, dtheta, painted) of the FP64 halo loop."""
+        halos = np.ascontiguousarray(halos, dtype=np.float32).reshape(-1, 7)
+        run = Run(0, 0, 10.0, zmin, zmax, chihview, model, profile, mmin, 1)
+        out = np.zeros((halos.shape[0], 6), np.float64)
+        self.L.p2mo_halo_setup(C.byref(run), C.byref(self.cosmo), _ptr(halos, C.c_float), halos.shape[0],
+                               _ptr(out, C.c_double))
         return out
 
     def paint_ref32(self, halos, nside=0, npix=0, fov_deg=10.0, zmin=0.05, zmax=1.2, chihview=0.0, model=1,

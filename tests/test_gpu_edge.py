@@ -190,3 +190,30 @@ def test_fortran_pks2map_scramble_and_centre(table, tmp_path, scramble, centre):
     nz = ref["hp"] != 0
     assert np.count_nonzero((hp != 0) != nz) == 0
     assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) < 1e-6
+
+
+def test_clamped_radius_cell_of_a_fine_radial_grid():
+    """A table whose radial step is smaller than ln(1 + 1e-5 / rmint): the
+    clamp r = rmint + 1e-5 (maptable.f90:190) lands in cell 2, not 1.  Nearly
+    every painted pixel has r < rmint here, so the FP32 painter must take the
+    clamped cell and fraction the reference forms (:196, :200); it matches the
+    oracle within 1e-6 with bit-exact membership."""
+    print("\n[gpu edge] clamped radius on a fine radial grid", flush=True)
+    nchit, nmht, nrt = 20, 10, 200
+    hi = np.array([nchit, nmht, nrt], np.int32)
+    hf = np.array([1e2, 7e3, 1e13, 1e16, 1.0, 1.00199], np.float32)
+    drt = (np.log(np.float64(hf[5])) - np.log(np.float64(hf[4]))) / (nrt - 1)
+    assert int(np.log(np.float64(hf[4]) + 1e-5) - np.log(np.float64(hf[4])) >= drt) == 1   # cell 2
+    ir = np.arange(1, nrt + 1, dtype=np.float64)
+    t = np.zeros((nchit, nmht, nrt, 2), np.float32)
+    for ichi in range(nchit):
+        for imh in range(nmht):
+            t[ichi, imh, :, 0] = -(1.0 + 0.5 * ir + 0.01 * imh + 0.003 * ichi) * 1e-7
+            t[ichi, imh, :, 1] = 1e-4
+    tab = (hi, hf, t.ravel())
+    h = catalog.config_catalog("C2", n=20000)
+    ref, hp, fs, (hc, hh, fc, fh), st = paint_both(h, tab, nside=1024, npix=256, prec=abi.PREC_FP32_COMP)
+    assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+    nz = ref["hp"] != 0
+    assert nz.sum() > 1000
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= 1e-6

@@ -28,3 +28,20 @@ def test_run_pieces_partition_the_run(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout
     assert "0 mismatches" in r.stdout
+
+
+def test_fast_ring_runs_equal_fp64_runs(tmp_path):
+    """disc_ring_run_fast (FP32 arc half-width, FP64 re-decide band) returns
+    the FP64 run of pixels_on_edge for random discs of radius 1e-6..2 rad at
+    nside 1..8192, polar centres included (the binning uses it for every
+    (halo, ring) item)."""
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not found")
+    exe = tmp_path / "runs_check"
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", "-ffp-contract=off",
+                    "-I", CSRC, os.path.join(ROOT, "tests", "runs_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    assert " 0 mismatches" in r.stdout
