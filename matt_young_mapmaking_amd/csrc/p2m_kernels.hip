@@ -1233,8 +1233,20 @@ struct FastSeg {
     uint32_t nb;
     uint32_t nfb;
     uint32_t lost;                 /* a deferred pixel of this tile found the list full */
-    uint8_t starts[4][64];         /* per wave: segment starts inside its current 64-pixel chunk */
+    uint32_t starts[4][64];        /* per wave: the chunk tag at each chunk pixel where a segment starts */
 };
+
+/* Orders a wave's LDS stores before the loads of other lanes of the same
+ * wave that follow.  A wave issues its LDS instructions in order, so the
+ * fences at wavefront scope emit no instruction; they stop the compiler from
+ * forwarding a lane's own store to its load (each lane is a thread to it, and
+ * a store of another lane is invisible without them). */
+__device__ static inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 /* sin^2(x) for |x| <= pi/2: the odd Taylor polynomial of sin to x^15 (the
  * next term is below 6e-12 there), squared; one branch-free sequence */
@@ -1345,7 +1357,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
     uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
     for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
     if (tid == 0) { sl->nfb = 0; sl->lost = 0; }
-    uint8_t *const starts = sl->starts[wv];
+    uint32_t *const starts = sl->starts[wv];
+    uint32_t tag = 0;                      /* chunks this wave has taken: a fresh tag per chunk */
+    starts[lane] = 0xffffffffu;
     lds_barrier();
 
     int64_t tbeg, tend, tstep;
@@ -1425,8 +1439,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             b0 += (uint64_t)nb;
             /* waves take 64-pixel chunks of the batch; a lane's segment is the
              * chunk's first segment plus the segments starting in the chunk up
-             * to its pixel (marked in the wave's LDS byte map, counted by ballot) */
-            for (uint32_t c0 = (uint32_t)wv * 64; c0 < P; c0 += (uint32_t)nwv * 64) {
+             * to its pixel: lane j writes the chunk's tag where segment sa+1+j
+             * starts (the starts are distinct: every staged segment has pixels),
+             * each lane reads its own pixel's entry, a ballot counts them.
+             * Entries of earlier chunks hold older tags. */
+            for (uint32_t c0 = (uint32_t)wv * 64; c0 < P; c0 += (uint32_t)nwv * 64, ++tag) {
                 int sa = 0, sh = nb - 1;
                 while (sa < sh) {
                     const int mid = (sa + sh + 1) >> 1;
@@ -1434,9 +1451,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 }
                 const int si = sa + 1 + lane;
                 const uint32_t st = si < nb ? sl->pre[si] : 0xffffffffu;
-                starts[lane] = 0;
-                if (st < c0 + 64) starts[st - c0] = 1;       /* st > c0: sa is the last start <= c0 */
-                const bool mine = starts[lane] != 0;
+                wave_lds_sync();                             /* the previous chunk's reads come first */
+                if (st < c0 + 64) starts[st - c0] = tag;     /* st > c0: sa is the last start <= c0 */
+                wave_lds_sync();
+                const bool mine = starts[lane] == tag;
                 const unsigned long long smask = __ballot(mine);
                 const uint32_t p = c0 + (uint32_t)lane;
                 if (p >= P) continue;

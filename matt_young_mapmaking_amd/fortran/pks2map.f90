@@ -391,26 +391,27 @@ contains
     end if
   end subroutine host_branches
 
-  ! 48-bit LCG in four 12-bit limbs (RandomField/random.f90:40-57, 264-286)
+  ! The 48-bit multiplicative congruential generator of ranf
+  ! (RandomField/random.f90:40-57, 264-286) in closed form: the draw is the
+  ! state X / 2^48, X read from the four 12-bit limbs of seed (which the
+  ! scramble seed holds unreduced on the first draw, quirk Q10), and the next
+  ! state is (A X) mod 2^48 for the multiplier A with limbs 373, 3707, 1442,
+  ! 647, formed from 24-bit halves so every product fits 64 bits.
   subroutine ranf(seed, r)
     integer, intent(inout) :: seed(4)
     double precision, intent(out) :: r
-    integer, parameter :: mult(4) = (/373, 3707, 1442, 647/)
-    integer :: j1, j2, j3, j4, k1, k2, k3, k4
-    r = real(seed(4)) / 4096.0d0 + real(seed(3)) / 16777216.0d0 + &
-        real(seed(2)) / 68719476736.0d0 + real(seed(1)) / 281474976710656.0d0
-    j1 = mult(1) * seed(1)
-    j2 = mult(1) * seed(2) + mult(2) * seed(1)
-    j3 = mult(1) * seed(3) + mult(2) * seed(2) + mult(3) * seed(1)
-    j4 = mult(1) * seed(4) + mult(2) * seed(3) + mult(3) * seed(2) + mult(4) * seed(1)
-    k1 = j1
-    k2 = j2 + k1 / 4096
-    k3 = j3 + k2 / 4096
-    k4 = j4 + k3 / 4096
-    seed(1) = mod(k1, 4096)
-    seed(2) = mod(k2, 4096)
-    seed(3) = mod(k3, 4096)
-    seed(4) = mod(k4, 4096)
+    integer(8), parameter :: b12 = 4096_8, b24 = 16777216_8, b48 = 281474976710656_8
+    integer(8), parameter :: a = 373_8 + b12 * (3707_8 + b12 * (1442_8 + b12 * 647_8))
+    integer(8) :: x, xl, xh, p
+    x = int(seed(1), 8) + b12 * (int(seed(2), 8) + b12 * (int(seed(3), 8) + b12 * int(seed(4), 8)))
+    r = dble(x) / dble(b48)
+    xl = mod(x, b24)
+    xh = x / b24
+    p = mod(mod(a, b24) * xl + b24 * mod(mod(a, b24) * xh + (a / b24) * xl, b24), b48)
+    seed(1) = int(mod(p, b12))
+    seed(2) = int(mod(p / b12, b12))
+    seed(3) = int(mod(p / b24, b12))
+    seed(4) = int(p / (b24 * b12))
   end subroutine ranf
 
 end program pks2map

@@ -8,6 +8,11 @@ oracle to finish at that size:
   * C5 (10^8 halos, nside 8192, six frequency tables in one pass): the same
     1e-6 bound and zero pattern, map by map.
 
+Each runs at the configs' light mass range [1e12, 10^15.5] (about 11% of
+the halos painted) and at the heavy range [1e13, 1e15] (about 87% painted,
+7x the pixel updates: 9.6e9 at C4, 3.9e10 per map at C5, (halo, ring) items
+beyond 2^31), where deferred pixels and segment lists are largest.
+
 The FP64 tiled painter's own parity with the oracle (1e-12, bit-exact
 membership) is tested at sizes the oracle finishes in seconds.
 """
@@ -33,8 +38,7 @@ def paint(dev, n, nside, tabs, prec, membership, tile, chunk=0):
     return p
 
 
-def compare(p32, p64, k):
-    a = p32.maps(slot=k, dtype=np.float32)[0]
+def compare(a, p64, k):
     b = p64.maps(slot=k, dtype=np.float64)[0]
     nz = b != 0
     rel = float(np.max(np.abs(a[nz].astype(np.float64) - b[nz]) / np.abs(b[nz])))
@@ -42,12 +46,15 @@ def compare(p32, p64, k):
     return rel, zm
 
 
-@pytest.mark.parametrize("wl", ["C4", "C5"])
-def test_fullsize_fp32_against_fp64(wl):
+@pytest.mark.parametrize("wl,masses", [("C4", "light"), ("C5", "light"), ("C4", "heavy"), ("C5", "heavy")])
+def test_fullsize_fp32_against_fp64(wl, masses):
     cfg = catalog.CONFIGS[wl]
     n, nside = cfg["n"], cfg["nside"]
-    print(f"\n[gpu fullsize] {wl}: {n} halos, nside {nside}", flush=True)
-    h = catalog.config_catalog(wl)
+    print(f"\n[gpu fullsize] {wl} {masses}: {n} halos, nside {nside}", flush=True)
+    if masses == "light":
+        h = catalog.config_catalog(wl)
+    else:
+        h = catalog.make_catalog(n, geom="sky", mlo=1e13, mhi=1e15)
     dev = devmem.DeviceArray(h)
     del h
     freqs = TABLES_C5 if wl == "C5" else [(148.0, 0.0)]
@@ -55,21 +62,29 @@ def test_fullsize_fp32_against_fp64(wl):
     tile = (1024, 256) if len(tabs) == 1 else (512, 256)
     member = wl == "C4"
     chunk = (1 << 24) if wl == "C4" else 0       # C4 is defined as 2^24-halo batches
+    # the FP32 maps go to the host before the FP64 context is made: at C5 each
+    # context holds about 100 GB of maps and binning buffers
     p32 = paint(dev, n, nside, tabs, abi.PREC_FP32_COMP, member, tile, chunk)
-    print(f"[gpu fullsize] {wl} fp32 painted", flush=True)
+    s32 = p32.stats()
+    print(f"[gpu fullsize] {wl} {masses} fp32 painted: {s32['n_upd_hp']} updates, {s32['n_segments']} segments, "
+          f"{s32['n_fallback']} deferred, {s32['n_redo_tiles']} tiles repainted", flush=True)
+    m32 = [p32.maps(slot=k, dtype=np.float32)[0] for k in range(len(tabs))]
+    if member:
+        c32, h32, _, _ = p32.membership()
+    p32.close()
     p64 = paint(dev, n, nside, tabs, abi.PREC_FP64, member, tile, chunk)
-    print(f"[gpu fullsize] {wl} fp64 painted", flush=True)
-    s32, s64 = p32.stats(), p64.stats()
+    s64 = p64.stats()
+    print(f"[gpu fullsize] {wl} {masses} fp64 painted", flush=True)
     assert s32["n_upd_hp"] == s64["n_upd_hp"] and s32["n_painted"] == s64["n_painted"]
     assert s32["n_ring_gaps"] == 0 and s32["n_radius_err"] == 0
     if member:
-        c32, h32, _, _ = p32.membership()
         c64, h64, _, _ = p64.membership()
         assert np.array_equal(c32, c64) and np.array_equal(h32, h64)
         assert int(c32.sum(dtype=np.uint64)) == s32["n_upd_hp"]
+        del c32, h32, c64, h64
     for k in range(len(tabs)):
-        rel, zm = compare(p32, p64, k)
-        print(f"[gpu fullsize] {wl} map {k}: max rel err {rel:.3e}, zero mismatch {zm}", flush=True)
+        rel, zm = compare(m32[k], p64, k)
+        print(f"[gpu fullsize] {wl} {masses} map {k}: max rel err {rel:.3e}, zero mismatch {zm}", flush=True)
         assert rel <= 1e-6 and zm == 0
-    p32.close()
     p64.close()
+    dev.close()
