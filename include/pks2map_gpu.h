@@ -175,6 +175,16 @@ int p2m_set_tile(p2m_ctx *ctx, int32_t tile_w, int32_t tile_wg);
  * dP/dy histogram (10000 bins of 1e-8, quirk Q13 binning). */
 int p2m_products(p2m_ctx *ctx, int slot, double moments[6], double rms[2], float dpdy[10000]);
 
+/* p2m_products with no allocation at all (no hipMalloc, hipFree or host
+ * heap): it has no counterpart in the reference.  It is for callers that
+ * cannot allocate -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation.  Its partial sums, dP/dy histogram and
+ * cumulative dP/dy live in a fixed 128 KiB device array per GPU, held until
+ * the process ends; no input changes that size (the histogram has 10000 bins
+ * and the moments 8 partial sums for any map).  Calls that overlap on one
+ * GPU share the array and race: serialise them. */
+int p2m_products_noalloc(p2m_ctx *ctx, int slot, double moments[6], double rms[2], float dpdy[10000]);
+
 /* Writers of pks2map.f90:359-381: <base>_fs.map, <base>_hp.fits, <base>_dp.bin
  * for map slot `slot` (file layouts of the reference; FITS written without
  * cfitsio).  Any of the three may be skipped by passing write_* = 0. */

@@ -24,7 +24,7 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "size mismatch", -3: "HIP error",
 EXPORTS = ["p2m_create", "p2m_destroy", "p2m_strerror", "p2m_set_run", "p2m_rofz", "p2m_zofr", "p2m_load_table",
            "p2m_paint", "p2m_paint_device", "p2m_host_alloc", "p2m_host_free", "p2m_stream_begin",
            "p2m_stream_push", "p2m_stream_finish", "p2m_clear", "p2m_fetch_maps", "p2m_fetch_membership", "p2m_get_stats",
-           "p2m_set_tile", "p2m_products", "p2m_write_outputs", "p2m_make_maptable", "p2m_selftest_math",
+           "p2m_set_tile", "p2m_products", "p2m_products_noalloc", "p2m_write_outputs", "p2m_make_maptable", "p2m_selftest_math",
            "p2m_selftest_copy", "p2m_version"]
 
 
@@ -89,6 +89,7 @@ def load():
     L.p2m_get_stats.argtypes = [vp, P(Stats)]
     L.p2m_set_tile.argtypes = [vp, i32, i32]
     L.p2m_products.argtypes = [vp, C.c_int, P(d), P(d), P(C.c_float)]
+    L.p2m_products_noalloc.argtypes = [vp, C.c_int, P(d), P(d), P(C.c_float)]
     L.p2m_write_outputs.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int]
     L.p2m_make_maptable.argtypes = [i32, i32, d, d, i32, i32, i32, P(i32), P(C.c_float), P(C.c_float), i64]
     L.p2m_version.restype = C.c_char_p
@@ -242,6 +243,16 @@ class Painter:
         dp = np.zeros(10000, np.float32)
         _chk(self.L.p2m_products(self.ctx, slot, _p(mom, C.c_double), _p(rms, C.c_double), _p(dp, C.c_float)),
              "products")
+        return mom, rms, dp
+
+    def products_noalloc(self, slot=0):
+        """p2m_products through the entry point that allocates nothing (its
+        scratch is a fixed device array per GPU)."""
+        mom = np.zeros(6, np.float64)
+        rms = np.zeros(2, np.float64)
+        dp = np.zeros(10000, np.float32)
+        _chk(self.L.p2m_products_noalloc(self.ctx, slot, _p(mom, C.c_double), _p(rms, C.c_double),
+                                         _p(dp, C.c_float)), "products_noalloc")
         return mom, rms, dp
 
     def write_outputs(self, base, slot=0, fs=True, hp=True, dp=True):

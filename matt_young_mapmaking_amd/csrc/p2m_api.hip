@@ -15,11 +15,42 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/pks2map_gpu.h"
 #include "p2m_kernels.h"
+
+/* Scratch of p2m_products_noalloc: one array per GPU, kept for the life of
+ * the process.  Layout: 8 doubles of partial sums, the 10000-bin dP/dy
+ * histogram (u64), the cumulative dP/dy (float).  Its device address comes
+ * from hipGetSymbolAddress once per device (the array's name is a host-side
+ * stand-in in host code). */
+#define P2M_NOALLOC_BYTES (128 * 1024)
+#define P2M_NOALLOC_HIST_OFF 64
+#define P2M_NOALLOC_DPDY_OFF (64 + 8 * 10000)
+static_assert(P2M_NOALLOC_DPDY_OFF + 4 * 10000 <= P2M_NOALLOC_BYTES, "no-alloc scratch layout");
+static __device__ __align__(256) unsigned char scratch[P2M_NOALLOC_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+/* the device address of this device's scratch (the device must be current) */
+static int scratch_for_device(int device, void **out)
+{
+    if (device < 0 || device >= 64) return P2M_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return P2M_E_HIP;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
 
 /* last HIP failure of this thread, reported by p2m_strerror(P2M_E_HIP) */
 static thread_local char g_hip_detail[256] = "";
@@ -295,6 +326,12 @@ int p2m_create(p2m_ctx **out, const p2m_config *cfg)
         return P2M_E_HIP;
     }
     for (int i = 0; i < 10; ++i) (void)hipEventCreate(&c->ev[i]);
+    {
+        /* this device's scratch of p2m_products_noalloc, resolved once */
+        void *sp = nullptr;
+        int e = scratch_for_device(cfg->device, &sp);
+        if (e) { p2m_destroy(c); return e; }
+    }
     if (hipHostMalloc((void **)&c->h_pin, sizeof(unsigned long long) * (Ctr::ALL + 8), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         c->h_pin = nullptr;
@@ -1377,6 +1414,57 @@ int p2m_products(p2m_ctx *c, int slot, double moments[6], double rms[2], float d
     }
     dfree(d_out); dfree(d_hist);
     if (rc) return rc;
+    if (moments) { for (int k = 0; k < 3; ++k) { moments[k] = res[0][k]; moments[3 + k] = res[1][k]; } }
+    if (rms) { rms[0] = res[0][3]; rms[1] = res[1][3]; }
+    return 0;
+}
+
+/* p2m_products without any allocation: the same kernels with the partial
+ * sums, the histogram and the cumulative dP/dy in this device's scratch array.
+ * Calls that overlap on one GPU share that array and race. */
+int p2m_products_noalloc(p2m_ctx *c, int slot, double moments[6], double rms[2], float dpdy[10000])
+{
+    if (!c) return P2M_E_INVALID;
+    if (c->in_stream) return P2M_E_STATE;
+    if (slot < 0 || slot >= c->cfg.n_maps) return P2M_E_INVALID;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    void *base = nullptr;
+    { int rc = scratch_for_device(c->cfg.device, &base); if (rc) return rc; }
+    double *d_out = (double *)base;
+    unsigned long long *d_hist = (unsigned long long *)((char *)base + P2M_NOALLOC_HIST_OFF);
+    float *d_dp = (float *)((char *)base + P2M_NOALLOC_DPDY_OFF);
+    { int rc = ensure_hp_valid(c); if (rc) return rc; }
+    double res[2][4] = {{0}};
+    const int ndpdy = 10000;
+    const float dy = 1e-4f / (float)ndpdy;                 /* pks2map.f90:39-40 */
+    for (int which = 0; which < 2; ++which) {
+        const int64_t n = which == 0 ? c->fsnpix : c->hpnpix;
+        if (n <= 0) continue;
+        const float *m32 = which == 1 ? c->d_hp32[slot] : nullptr;
+        const double *m64 = which == 1 ? c->d_hp64[slot] : c->d_fs64[slot];
+        if (m32) m64 = nullptr;
+        /* the scratch keeps the last call's contents: cleared by fill kernels */
+        p2m_launch_fill_f64(d_out, 8, c->stream);
+        if (which == 1) p2m_launch_fill_u64(d_hist, ndpdy, c->stream);
+        p2m_launch_moments(m32, m64, n, 0.0, d_out, which == 1 ? d_hist : nullptr, ndpdy, dy, c->stream);
+        double h1[8];
+        HIPCHK(hipMemcpyAsync(h1, d_out, sizeof(h1), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const double mean = h1[0] / (double)n;
+        p2m_launch_fill_f64(d_out, 8, c->stream);
+        p2m_launch_moments(m32, m64, n, mean, d_out, nullptr, ndpdy, dy, c->stream);
+        double h2[8];
+        HIPCHK(hipMemcpyAsync(h2, d_out, sizeof(h2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        res[which][0] = h1[0] / (double)n; res[which][1] = h1[1] / (double)n; res[which][2] = h1[2] / (double)n;
+        res[which][3] = p2m_sqrt(h2[3] / (double)n);
+        if (which == 1 && dpdy) {
+            p2m_launch_dpdy_cumulative(d_hist, ndpdy, (float)n, d_dp, c->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(dpdy, d_dp, sizeof(float) * ndpdy, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    }
     if (moments) { for (int k = 0; k < 3; ++k) { moments[k] = res[0][k]; moments[3 + k] = res[1][k]; } }
     if (rms) { rms[0] = res[0][3]; rms[1] = res[1][3]; }
     return 0;

@@ -182,6 +182,7 @@ int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int
 int p2m_launch_fallback(const FallbackArgs &a, int64_t n_max, hipStream_t s);
 void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double mean, double *out,
                         unsigned long long *hist, int nb, float dy, hipStream_t s);
+void p2m_launch_dpdy_cumulative(const unsigned long long *hist, int nb, float npix, float *dpdy, hipStream_t s);
 void p2m_launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s);
 void p2m_launch_f32_to_f64(const float *in, double *out, int64_t n, hipStream_t s);
 

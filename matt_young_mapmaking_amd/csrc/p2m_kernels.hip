@@ -1783,6 +1783,24 @@ void p2m_launch_moments(const float *m32, const double *m64, int64_t n, double m
     hipLaunchKernelGGL(k_moments, dim3(fill_grid(n) < 2048 ? fill_grid(n) : 2048), dim3(256), 0, s, m32, m64, n, mean, out, hist, nb, dy);
 }
 
+/* dpdy[i] = hist[i] / npix, then the cumulative sum, in float and in order
+ * (pks2map.f90:347, 355-357), by one thread */
+__global__ void k_dpdy_cumulative(const unsigned long long *hist, int nb, float npix, float *dpdy)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float run = 0.0f;
+    for (int i = 0; i < nb; ++i) {
+        const float v = (float)hist[i] / npix;
+        run = i == 0 ? v : run + v;
+        dpdy[i] = run;
+    }
+}
+
+void p2m_launch_dpdy_cumulative(const unsigned long long *hist, int nb, float npix, float *dpdy, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dpdy_cumulative, dim3(1), dim3(64), 0, s, hist, nb, npix, dpdy);
+}
+
 __global__ void k_f64_to_f32(const double *in, float *out, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
