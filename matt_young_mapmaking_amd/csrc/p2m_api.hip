@@ -221,15 +221,12 @@ static int build_tiles(p2m_ctx *c, int log2w)
     int32_t ns = c->cfg.nside;
     if (ns <= 0) return 0;
     if (c->log2w == log2w && c->d_tile_base) return 0;
-    int W = 1 << log2w;
     c->n_rings = 4 * ns - 1;
     c->tile_base.assign(c->n_rings, 0);
     int64_t acc = 0;
     for (int r = 1; r <= c->n_rings; ++r) {
-        int32_t m = std::min(std::min(ns, r), 4 * ns - r);
-        int32_t npr = 4 * m;
         c->tile_base[r - 1] = (int32_t)acc;
-        acc += (npr + W - 1) / W;
+        acc += ring_tiles(ns, r, log2w);
     }
     c->n_tiles = acc;
     std::vector<int32_t> tile_ring((size_t)acc);

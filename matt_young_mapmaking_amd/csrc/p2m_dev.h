@@ -146,7 +146,7 @@ P2M_DEV double ring2z(int32_t nside, int32_t ir)
 }
 
 /* pixels_per_ring, pix_tools.F90:802-836: npr, kshift, first pixel of ring */
-P2M_DEV void ring_info(int32_t nside, int32_t ring, int32_t &npr, int32_t &kshift, int64_t &first)
+P2M_HD void ring_info(int32_t nside, int32_t ring, int32_t &npr, int32_t &kshift, int64_t &first)
 {
     int32_t m = nside < ring ? nside : ring;
     if (4 * nside - ring < m) m = 4 * nside - ring;
@@ -164,6 +164,29 @@ P2M_DEV void ring_info(int32_t nside, int32_t ring, int32_t &npr, int32_t &kshif
         npnorth = 12LL * nside * nside - irs * (irs + 1LL) * 2LL;
     }
     first = npnorth - npr;
+}
+
+/* Ring tiles: ring `ring` is cut into ceil(npr / W) tiles of W = 2^log2w
+ * consecutive RING pixels, the last one narrower.  Tile `col` starts at
+ * in-ring pixel ip0 = col W, holds `width` pixels, and its first map pixel is
+ * first + ip0 (first: the ring's first pixel).  The host numbers the tiles
+ * ring by ring with ring_tiles; k_tile_desc fills each descriptor with
+ * tile_span. */
+P2M_HD int32_t ring_tiles(int32_t nside, int32_t ring, int32_t log2w)
+{
+    int32_t npr, kshift;
+    int64_t first;
+    ring_info(nside, ring, npr, kshift, first);
+    return (npr + (1 << log2w) - 1) >> log2w;
+}
+
+P2M_HD void tile_span(int32_t nside, int32_t ring, int32_t col, int32_t log2w, int64_t &first, int32_t &ip0,
+                      int32_t &width)
+{
+    int32_t npr, kshift;
+    ring_info(nside, ring, npr, kshift, first);
+    ip0 = col << log2w;
+    width = npr - ip0 < (1 << log2w) ? npr - ip0 : (1 << log2w);
 }
 
 /* Pixel-centre z and sin(theta) of ring `ring`, exactly as pix2vec_ring

@@ -1,4 +1,6 @@
-"""The enumeration of a ring run's tile pieces (p2m_pieces.h), which the
+"""The ring tiles (ring_tiles, tile_span in p2m_dev.h) stay inside the map
+for every nside 1..8192 and tile width 64..2048.  The enumeration of a ring
+run's tile pieces (p2m_pieces.h), which the
 binning uses to cut every (halo, ring) run at the tile boundaries: each piece
 lies in one tile, the pieces follow the run from its first pixel in ring
 order, wrapping past the ring's end once, and cover it exactly; the emit
@@ -41,6 +43,23 @@ def test_fast_ring_runs_equal_fp64_runs(tmp_path):
     exe = tmp_path / "runs_check"
     subprocess.run([hipcc, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", "-ffp-contract=off",
                     "-I", CSRC, os.path.join(ROOT, "tests", "runs_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    assert " 0 mismatches" in r.stdout
+
+
+def test_ring_tiles_stay_inside_the_map(tmp_path):
+    """Every tile of every ring, for nside 1..8192 and W = 64..2048: its
+    first map pixel and width keep it inside its ring and the map, and the
+    tiles of a ring cover it exactly (the bounds the tile painter's stores
+    rely on)."""
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not found")
+    exe = tmp_path / "tiles_check"
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "tiles_check.cpp"), "-o", str(exe)],
                    check=True, capture_output=True, timeout=300)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout
