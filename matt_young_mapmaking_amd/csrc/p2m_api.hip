@@ -167,6 +167,7 @@ struct p2m_ctx {
     /* per tile: 1 if the tile's deferred pixels did not all fit in the list
      * (the tile then wrote nothing and is painted again) */
     uint8_t *d_lost = nullptr;
+    unsigned int *d_tile_next = nullptr;   /* tile-queue counters of the FP32 painter */
     int n_redo = 0;
     /* the FP32 painter's last launch, kept for the repaint pass */
     TileArgs last_t;
@@ -240,6 +241,7 @@ static int build_tiles(p2m_ctx *c, int log2w)
     c->d_tile_ring = nullptr; c->d_tile_desc = nullptr; c->d_lost = nullptr;
     int rc = 0;
     rc |= dmalloc((void **)&c->d_lost, (size_t)((acc + 3) / 4) * 4);
+    if (!c->d_tile_next) rc |= dmalloc((void **)&c->d_tile_next, sizeof(unsigned int) * 8 * P2M_TQ_STRIDE);
     rc |= dmalloc((void **)&c->d_tile_ring, sizeof(int32_t) * acc);
     rc |= dmalloc((void **)&c->d_tile_desc, sizeof(TileDesc) * acc);
     rc |= dmalloc((void **)&c->d_tile_base, sizeof(int32_t) * c->n_rings);
@@ -368,7 +370,7 @@ void p2m_destroy(p2m_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fs_stream) (void)hipStreamSynchronize(c->fs_stream);
     if (c->h2d_stream) (void)hipStreamSynchronize(c->h2d_stream);
-    dfree(c->d_slot[0]); dfree(c->d_slot[1]); dfree(c->d_lost);
+    dfree(c->d_slot[0]); dfree(c->d_slot[1]); dfree(c->d_lost); dfree(c->d_tile_next);
     for (int i = 0; i < 2; ++i) {
         if (c->ev_copy[i]) (void)hipEventDestroy(c->ev_copy[i]);
         if (c->ev_free[i]) (void)hipEventDestroy(c->ev_free[i]);
@@ -872,7 +874,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         t.inv_node2f = c->d_inv_node2f;
         t.n_quad = c->n_quad; t.tab_q = c->d_quad;
         t.fb = c->d_fb; t.fb_count = pc + 6; t.fb_cap = c->fb_cap;
-        t.lost = c->d_lost; t.redo = 0; t.n_halo = n_halo; t.err = c->d_ctr + Ctr::ERR; t.n_lost = c->d_ctr + Ctr::LOST;
+        t.lost = c->d_lost; t.tile_next = c->d_tile_next; t.redo = 0; t.n_halo = n_halo; t.err = c->d_ctr + Ctr::ERR; t.n_lost = c->d_ctr + Ctr::LOST;
         for (int m = 0; m < c->cfg.n_maps; ++m) {
             t.tab_flat[m] = c->d_flat[m]; t.map64[m] = c->d_hp64[m]; t.map32[m] = c->d_hp32[m];
         }

@@ -94,6 +94,8 @@ struct __attribute__((aligned(16))) TileDesc {
     int32_t ring, ip0, width, pad;
 };
 
+#define P2M_TQ_STRIDE 32           /* tile-queue counters one 128-byte line apart */
+
 struct TileArgs {
     const HaloRec *halos;
     const TileHalo *hot;          /* the same halos, packed for the FP32 painter */
@@ -126,6 +128,8 @@ struct TileArgs {
     int32_t redo;                 /* paint only the tiles marked lost, clearing the mark of each
                                      that now fits */
     int64_t n_halo;               /* records in hot / halos */
+    unsigned int *tile_next;      /* the FP32 painter's tile queues: one counter per XCD, P2M_TQ_STRIDE
+                                     apart (zeroed by p2m_launch_tile_paint) */
     unsigned long long *counters; /* [1] hp updates, [5] FP64 fallbacks */
     unsigned long long *err;      /* sharded: segments whose halo or pixel range is out of bounds */
     unsigned long long *n_lost;   /* sharded: tiles marked lost */

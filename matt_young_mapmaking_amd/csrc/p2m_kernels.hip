@@ -5,16 +5,29 @@
  *                    then the FP64 per-halo setup of the survivors
  *                    (pks2map.f90:144-160, 264, 284-298; haloproject.f90:47-55,
  *                    93-128; maptable.f90:184-202; query_disc prologue)
+ *  k_gather_halos    the survivors in spatial order (band of rings x azimuth),
+ *                    packed for the binning and the FP32 painter
+ *  k_ring_counts, k_group_heads, k_seg_count, k_seg_emit
+ *                    bin the (halo, ring) pixel runs of query_disc into ring
+ *                    tiles of W pixels (pix_tools.F90:712-800,
+ *                    pixel_routines.F90:1823-1879)
+ *  k_tile_fast       FP32 painter: workgroups take ring tiles from a queue per
+ *                    XCD; per pixel the FP32 chord^2 test and profile value,
+ *                    summed in an FP64 LDS tile and written once as FP32;
+ *                    pixels near a disc edge or a table-cell edge go to
+ *  k_fallback        the exact FP64 decision and value (haloproject.f90:60-75,
+ *                    maptable.f90:176-216)
+ *  k_tile_exact      FP64 painter over the same tiles, the reference's
+ *                    arithmetic per pixel, FP64 LDS tile
  *  k_paint_plain_hp  baseline: one work item per halo, naive query_disc loop,
  *                    pix2vec_ring/angdist/interpolate_table per pixel, FP64
  *                    global atomics (haloproject.f90:55-75)
- *  k_paint_fs        flat-sky painter, one wave per halo (haloproject.f90:136-152)
- *  k_seg_count/emit  bin the (halo, ring) pixel runs of query_disc into ring
- *                    tiles of W pixels (pix_tools.F90:712-800,
- *                    pixel_routines.F90:1823-1879)
- *  k_tile_paint      one workgroup per ring tile: every halo segment of the
- *                    tile painted into an LDS tile (FP64, or FP32 two-sum),
- *                    then one FP64 tile-to-map reduce per pixel
+ *  k_fs_counts, k_fs_paint, k_paint_fs
+ *                    flat-sky painter over the flattened (halo, box pixel)
+ *                    space; one work item per halo for the baseline
+ *                    (haloproject.f90:136-152)
+ *  k_moments, k_dpdy_cumulative
+ *                    map moments and the dP/dy histogram (pks2map.f90:321-357)
  */
 #include "p2m_kernels.h"
 #include "p2m_pieces.h"
@@ -1228,6 +1241,7 @@ struct FastSeg {
     uint32_t nfb;
     uint32_t lost;                 /* a deferred pixel of this tile found the list full */
     unsigned long long fb_base;    /* list slot of the tile's deferred pixels */
+    int32_t next_tile;             /* the tile this workgroup paints next */
     uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
     uint16_t lo[FSEG];             /* first in-tile pixel (< 2048) */
@@ -1366,20 +1380,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
     for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
     lds_barrier();
 
-    int64_t tbeg, tend, tstep;
-    tile_range(A, tbeg, tend, tstep);
-    /* the descriptor and segment range of the next tile are requested (as
-     * scalar loads: read-only, uniform) once the current tile's first
-     * segments are, so they arrive during its staging */
-    TileDesc Gn;
-    uint64_t sbn = 0, sen = 0;
-    if (tbeg < tend) { Gn = tdesc[tbeg]; sbn = toff[tbeg]; sen = toff[tbeg + 1]; }
-    for (int64_t t = tbeg; t < tend; t += tstep) {
-        const TileDesc G = Gn;
-        const uint64_t sbeg = sbn, send = sen;
-        bool pending = t + tstep < tend;
+    /* Tiles come from one queue per XCD: workgroup b runs on XCD b % 8 and
+     * claims the next tile of that XCD's contiguous eighth of the tiles
+     * (ring-major, so the tiles in flight on an XCD share halos in its L2)
+     * from the XCD's counter.  A workgroup that becomes resident late, or
+     * whose tiles run long, leaves the remaining tiles to the others.
+     * Thread 0 claims the following tile as a tile starts; the claim is
+     * published with the tile's last barrier. */
+    const bool xq = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
+    const int xcd = xq ? (int)(blockIdx.x % 8) : 0;
+    const int64_t xb = xq ? (int64_t)xcd * A.n_tiles / 8 : 0;
+    const int64_t xe = xq ? (int64_t)(xcd + 1) * A.n_tiles / 8 : A.n_tiles;
+    unsigned int *const qctr = A.tile_next + xcd * P2M_TQ_STRIDE;
+    if (tid == 0) sl->next_tile = (int32_t)(xb + (int64_t)atomicAdd(qctr, 1u));
+    lds_barrier();
+    for (;;) {
+        /* every thread reads the same tile: taken as uniform, its descriptor
+         * and segment range are scalar loads */
+        const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane(sl->next_tile);
+        if (t >= xe) break;
+        unsigned int claim = 0;
+        if (tid == 0) claim = atomicAdd(qctr, 1u);
+        const TileDesc G = tdesc[t];
+        const uint64_t sbeg = toff[t], send = toff[t + 1];
         if (A.redo && !A.lost[t]) {      /* repaint pass: only the tiles that lost pixels */
-            if (pending) { Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1]; }
+            lds_barrier();
+            if (tid == 0) sl->next_tile = (int32_t)(xb + (int64_t)claim);
+            lds_barrier();
             continue;
         }
         const double kphi = G.kphi;
@@ -1398,10 +1425,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
             const uint2 sg = tid < nstage ? A.segs[b0 + tid] : make_uint2(0u, 0u);
-            if (pending) {
-                pending = false;
-                Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
-            }
             uint32_t slo = 0;
             if (tid < nstage) {
                 /* a segment must name a record and lie inside the tile; one
@@ -1521,9 +1544,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 }
             }
         }
-        if (pending) {     /* a tile without segments */
-            Gn = tdesc[t + tstep]; sbn = toff[t + tstep]; sen = toff[t + tstep + 1];
-        }
         lds_barrier();
         /* the tile's deferred pixels still in LDS join the list before the
          * tile is written: a tile whose deferred pixels do not all fit in the
@@ -1567,9 +1587,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             }
         }
         lds_barrier();
-        /* every thread has read nfb and lost; the next tile's first batch
-         * starts with a barrier, which orders this reset before its deferrals */
-        if (tid == 0) { sl->nfb = 0; sl->lost = 0; }
+        /* every thread has read nfb and lost; the reset and the next tile
+         * are published by the barrier below */
+        if (tid == 0) { sl->nfb = 0; sl->lost = 0; sl->next_tile = (int32_t)(xb + (int64_t)claim); }
+        lds_barrier();
     }
     /* the staged constants are dead: their space takes the block sums (the
      * kernel has no static LDS, so its occupancy is what its launch asks) */
@@ -1729,6 +1750,10 @@ int p2m_launch_tile_paint(const TileArgs &a, int mode, int n_maps, int grid, int
 {
     size_t lds = p2m_tile_lds_bytes(mode, n_maps, a.log2w, a.count != nullptr, a.g.nrt);
     if (lds > 160 * 1024 || wg > 512 || (mode == 2 && wg > 256)) return -1;
+    if (mode == 2) {
+        if (!a.tile_next) return -1;
+        p2m_launch_fill_u32(a.tile_next, 8 * P2M_TQ_STRIDE, 0u, s);   /* empty tile queues */
+    }
 #define P2M_TE(NMv) case NMv: hipLaunchKernelGGL((k_tile_exact<NMv>), dim3(grid), dim3(wg), lds, s, a); break;
 #define P2M_TF(NMv) case NMv: hipLaunchKernelGGL((k_tile_fast<NMv>), dim3(grid), dim3(wg), lds, s, a, a.desc, a.tile_off); break;
     if (mode == 1) {
