@@ -33,6 +33,7 @@
 #include "p2m_pieces.h"
 
 #include <hipcub/hipcub.hpp>
+#include <type_traits>
 
 /* ===================================================================== */
 /* helpers                                                                */
@@ -1244,7 +1245,7 @@ struct FastSeg {
     int32_t next_tile;             /* the tile this workgroup paints next */
     uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
-    uint16_t lo[FSEG];             /* first in-tile pixel (< 2048) */
+    int16_t qoff[FSEG];            /* in-tile pixel minus batch pixel: lo - pre (-3072 .. 2047) */
     float A[FSEG], B[FSEG], thr_hi[FSEG], chi2[FSEG];   /* chord^2 = A + B sin^2(x); its band top; chi^2 */
     float w00[FSEG], w10[FSEG], w01[FSEG], w11[FSEG];   /* corner weights times the velocity factor */
     float xc[FSEG];                /* azimuth origin x_c */
@@ -1252,8 +1253,9 @@ struct FastSeg {
     int32_t colbase[FSEG];
 };
 
-/* sin^2(x) for |x| <= pi/2: the odd Taylor polynomial of sin to x^15 (the
- * next term is below 6e-12 there), squared; one branch-free sequence */
+/* sin^2(x) for |x| <= 2: the odd Taylor polynomial of sin to x^15 (the
+ * next term is below 3e-10 there, 6e-12 at pi/2), squared; one branch-free
+ * sequence */
 __device__ static inline float sin2f(float x)
 {
     const float z = x * x;
@@ -1414,7 +1416,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
         const float hphi = (float)(0.5 * kphi);          /* x step per pixel along the ring */
         const int32_t mr = G.ring < A.nside ? G.ring : (G.ring < 3 * A.nside ? A.nside : 4 * A.nside - G.ring);
         const int32_t npr = 4 * mr, half_npr = npr / 2;
-        unsigned long long tu = 0, tf = 0;   /* this tile's updates and deferrals */
+        uint32_t tu = 0, tf = 0;             /* this tile's updates and deferrals */
         for (int q = tid; q < G.width; q += blockDim.x) {
             for (int m = 0; m < NM; ++m) acc[m * W + q] = 0.0;
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
@@ -1440,7 +1442,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             if (tid < nb) {
                 const TileHalo &h = A.hot[sg.x < (uint64_t)A.n_halo ? sg.x : 0u];
                 sl->halo[tid] = sg.x;
-                sl->lo[tid] = (uint16_t)slo;
+                sl->qoff[tid] = (int16_t)((int32_t)slo - (int32_t)sl->pre[tid]);
                 const double dz = G.zr - h.zz0, ds = G.sth - h.s0;
                 sl->A[tid] = (float)(dz * dz + ds * ds);
                 sl->B[tid] = (float)((4.0 * G.sth) * h.s0);
@@ -1455,27 +1457,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 /* azimuth origin: the ring pixel ip_c nearest the halo centre's
                  * azimuth, with x_c = ((ip_c + off) kphi - phi0) / 2 folded by
                  * pi (|x_c| <= kphi / 4) in FP64; a pixel's x is then x_c plus
-                 * (ip - ip_c) steps of kphi / 2, its step count folded into
-                 * [-npr/2, npr/2] (sin^2 has period pi), so |x| <= pi/2 + kphi/4
-                 * and its FP32 error stays relative to x */
+                 * n = ip - ip_c steps of kphi / 2 in FP32.  n may move by npr
+                 * (x by pi: sin^2 has period pi): the segment's middle step is
+                 * moved into [-npr/2, npr/2] here, so on rings of at least 4 W
+                 * pixels every n of the segment is within npr/2 + W/2 and
+                 * |x| <= pi/2 + pi/8 + kphi/4; on shorter rings each pixel's n
+                 * is folded into [-npr/2, npr/2] as well.  Either way the FP32
+                 * error of x stays relative to x. */
                 double ipc = __builtin_rint(h.phi0 * inv_kphi - G.off);
                 if (ipc < 0.0) ipc += (double)npr;
                 if (ipc >= (double)npr) ipc -= (double)npr;
                 double xc = 0.5 * ((ipc + G.off) * kphi - h.phi0);
                 xc -= HPX_PI * __builtin_rint(xc * (1.0 / HPX_PI));
                 sl->xc[tid] = (float)xc;
-                sl->cidx[tid] = (int32_t)ipc - G.ip0 - ((int32_t)slo - (int32_t)sl->pre[tid]);
+                const int32_t nmid = G.ip0 + (int32_t)slo + (int32_t)(len >> 1) - (int32_t)ipc;
+                sl->cidx[tid] = (int32_t)ipc - G.ip0 - ((int32_t)slo - (int32_t)sl->pre[tid]) +
+                                (nmid > half_npr ? npr : (nmid < -half_npr ? -npr : 0));
                 if (A.count) hh[tid] = h.hash;
             }
             build_seg_map(sl, nb, P);
             lds_barrier();
             b0 += (uint64_t)nb;
             const unsigned char *segb = (const unsigned char *)sl->seg_of;
+            /* the pixel loop, with or without the per-pixel fold of n (a
+             * tile-uniform choice: two copies of the loop) */
+            auto paint_pixels = [&](auto fold) {
             for (uint32_t p = tid; p < P; p += blockDim.x) {
                 const int s = segb[p];
-                const int q = (int)(sl->lo[s] + (p - sl->pre[s]));
+                const int q = (int)p + sl->qoff[s];
                 int n = (int)p - sl->cidx[s];
-                n = n > half_npr ? n - npr : (n < -half_npr ? n + npr : n);
+                if constexpr (decltype(fold)::value) n = n > half_npr ? n - npr : (n < -half_npr ? n + npr : n);
                 const float x = __builtin_fmaf((float)n, hphi, sl->xc[s]);
                 const float s2 = sin2f(x);
                 const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
@@ -1543,6 +1554,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                     atomicAdd(&lhash[q], hh[s]);
                 }
             }
+            };
+            if (npr < 4 * W) paint_pixels(std::true_type{});
+            else paint_pixels(std::false_type{});
         }
         lds_barrier();
         /* the tile's deferred pixels still in LDS join the list before the
@@ -1560,7 +1574,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             if (lost) { A.lost[t] = 1; atomicAdd(A.n_lost + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE, 1ULL); }
             else if (A.redo) A.lost[t] = 0;
         }
-        if (!lost) { n_upd += tu; n_fb += tf; }
+        if (!lost) { n_upd += (unsigned long long)tu; n_fb += (unsigned long long)tf; }
         const int64_t pix0 = G.first + G.ip0;
         /* the branch on accumulate is taken outside the loops: a select
          * inside lets the compiler load the map speculatively (a map read and
