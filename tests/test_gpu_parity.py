@@ -193,3 +193,42 @@ def test_deferred_list_overflow_repaints(table, monkeypatch):
     assert s1["n_upd_hp"] + s2["n_upd_hp"] == ref["stats"]["n_upd_hp"]
     assert int(np.count_nonzero((hp != 0) != (ref["hp"] != 0))) == 0
     assert rel_err(hp, ref["hp"]) <= 1e-6
+
+
+TABLES_C5 = [(148.0, 0.0), (200.0, 0.0), (201.0, 0.0), (218.0, 0.0), (353.0, 0.1), (545.0, 0.0)]
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_COMP])
+def test_parity_nside8192_six_tables(prec):
+    """The C5 geometry (nside 8192, the six frequency tables of one pass, the
+    default 512-pixel tiles of six maps) against the oracle painting each
+    table alone, on 2e5 halos of the C5 catalogue: membership bit-exact, the
+    same tolerances map by map."""
+    print(f"\n[gpu parity] nside 8192, six tables, precision={prec}", flush=True)
+    nside, npix = 8192, 1024
+    h = catalog.config_catalog("C5", n=200_000)
+    tabs = [abi.make_maptable(model=1, nu=nu, alpha=a) for nu, a in TABLES_C5]
+    with abi.Painter(nside=nside, npix_fs=npix, precision=prec, n_maps=len(tabs), membership=True) as p:
+        p.set_run()
+        for k, t in enumerate(tabs):
+            p.load_table(k, *t)
+        p.paint(h)
+        hc, hh, fc, fh = p.membership()
+        st = p.stats()
+        tol, ytol = TOL[prec]
+        for k, t in enumerate(tabs):
+            o = p2mo.Oracle(*t)
+            ref = o.paint(h, nside=nside, npix=npix, nthreads=NTHREADS, membership=(k == 0))
+            if k == 0:
+                assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+                assert np.array_equal(fc, ref["fs_count"]) and np.array_equal(fh, ref["fs_hash"])
+                for key in ("n_cut", "n_painted", "n_listed_hp", "n_upd_hp", "n_upd_fs", "n_ring_gaps"):
+                    assert st[key] == ref["stats"][key], key
+            hp, fs = p.maps(slot=k)
+            assert int(np.count_nonzero((hp != 0) != (ref["hp"] != 0))) == 0
+            err = rel_err(hp, ref["hp"])
+            assert err <= tol, (k, err)
+            assert rel_err(fs, ref["fs"]) <= 1e-12
+            assert abs(hp.sum() - ref["hp"].sum()) <= ytol * abs(ref["hp"].sum())
+            print(f"[gpu parity] nside 8192 table {k}: max rel err {err:.3e}", flush=True)
+            del ref, hp, fs
