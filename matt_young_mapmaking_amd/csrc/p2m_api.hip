@@ -708,6 +708,13 @@ static int autotune(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long lo
 
 static int launch_flat(p2m_ctx *c, PaintArgs &pa, int64_t n_flat, hipStream_t s);
 
+/* the main stream waits for whatever the flat-sky stream has enqueued */
+static void fs_join(p2m_ctx *c)
+{
+    if (hipEventRecord(c->ev_fs, c->fs_stream) == hipSuccess) (void)hipStreamWaitEvent(c->stream, c->ev_fs, 0);
+    (void)hipGetLastError();
+}
+
 static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long long n_items, bool allow_tune)
 {
     const int mode = c->cfg.precision;
@@ -749,8 +756,15 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
          * its own stream beside the binning and the tile painter */
         HIPCHK(hipEventRecord(c->ev_dec, c->stream));
         HIPCHK(hipStreamWaitEvent(c->fs_stream, c->ev_dec, 0));
-        { int rc = launch_flat(c, pa, n_flat, c->fs_stream); if (rc) return rc; }
+        { int rc = launch_flat(c, pa, n_flat, c->fs_stream); if (rc) { fs_join(c); return rc; } }
         HIPCHK(hipEventRecord(c->ev_fs, c->fs_stream));
+        /* every return from here on, error or not, first orders the work
+         * that follows on the main stream after the flat-sky painter */
+        struct FsJoin {
+            p2m_ctx *c;
+            bool armed;
+            ~FsJoin() { if (armed) (void)hipStreamWaitEvent(c->stream, c->ev_fs, 0); }
+        } fs_guard{c, true};
         /* painted halos in spatial order (keys written by decode):
          * neighbouring halos share tiles */
         {
@@ -908,6 +922,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         }
         c->last_fast = mode == P2M_PREC_FP32_COMP;
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        fs_guard.armed = false;
         HIPCHK(hipStreamWaitEvent(c->stream, c->ev_fs, 0));
     } else {
         HIPCHK(hipEventRecord(c->ev[3], c->stream));

@@ -66,7 +66,8 @@ def test_fullsize_fp32_against_fp64(wl, masses):
     # context holds about 100 GB of maps and binning buffers
     p32 = paint(dev, n, nside, tabs, abi.PREC_FP32_COMP, member, tile, chunk)
     s32 = p32.stats()
-    print(f"[gpu fullsize] {wl} {masses} fp32 painted: {s32['n_upd_hp']} updates, {s32['n_segments']} segments, "
+    print(f"[gpu fullsize] {wl} {masses} fp32 painted in {s32['ms_total']:.1f} ms (tile kernel {s32['ms_tile']:.1f}): "
+          f"{s32['n_upd_hp']} updates, {s32['n_segments']} segments, "
           f"{s32['n_fallback']} deferred, {s32['n_redo_tiles']} tiles repainted", flush=True)
     m32 = [p32.maps(slot=k, dtype=np.float32)[0] for k in range(len(tabs))]
     if member:
@@ -74,7 +75,7 @@ def test_fullsize_fp32_against_fp64(wl, masses):
     p32.close()
     p64 = paint(dev, n, nside, tabs, abi.PREC_FP64, member, tile, chunk)
     s64 = p64.stats()
-    print(f"[gpu fullsize] {wl} {masses} fp64 painted", flush=True)
+    print(f"[gpu fullsize] {wl} {masses} fp64 painted in {s64['ms_total']:.1f} ms", flush=True)
     assert s32["n_upd_hp"] == s64["n_upd_hp"] and s32["n_painted"] == s64["n_painted"]
     assert s32["n_ring_gaps"] == 0 and s32["n_radius_err"] == 0
     if member:
