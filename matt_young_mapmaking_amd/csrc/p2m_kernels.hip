@@ -135,6 +135,7 @@ __device__ static inline void block_count(unsigned long long *counter, unsigned 
 /* ===================================================================== */
 
 #define CULL_SPAN 8192
+#define CULL_U 4                   /* records per thread in flight in k_cull */
 
 /* Cull pass over every record: the radial cut (pks2map.f90:144-160) and the
  * mass cut on the unconverted mass (:264, :296).  A workgroup takes spans of
@@ -158,26 +159,37 @@ __global__ void __launch_bounds__(256) k_cull(DecodeArgs A)
         if (threadIdx.x == 0) nbuf = 0;
         __syncthreads();
         const int64_t i0 = sp * CULL_SPAN;
-        for (int k = threadIdx.x; k < CULL_SPAN; k += blockDim.x) {
-            const int64_t i = i0 + k;
-            bool take = false;
-            if (i < A.n) {
-                const float *p = A.halos + 7 * i;
-                const double xh = (double)p[2], yh = (double)p[1], zh = (double)p[0];
-                const double xs = xh - A.chihview;
-                const double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
-                if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
-                    tot_cut++;
-                    const double mass = mconst * cube((double)p[6]);
-                    take = !(mass < A.mmin);
-                }
+        /* CULL_U records per thread per step, all four fields of each loaded
+         * before any is used: the loads of a step are in flight together */
+        for (int kb = 0; kb < CULL_SPAN; kb += CULL_U * (int)blockDim.x) {
+            float r0[CULL_U], r1[CULL_U], r2[CULL_U], r6[CULL_U];
+#pragma unroll
+            for (int u = 0; u < CULL_U; ++u) {
+                const int64_t i = i0 + kb + u * (int)blockDim.x + (int)threadIdx.x;
+                const float *p = A.halos + 7 * (i < A.n ? i : 0);
+                r0[u] = p[0]; r1[u] = p[1]; r2[u] = p[2]; r6[u] = p[6];
             }
-            const unsigned long long mask = __ballot(take);
-            if (mask) {
-                unsigned int off = 0;
-                if (lane == 0) off = atomicAdd(&nbuf, (unsigned int)__popcll(mask));
-                off = __shfl(off, 0);
-                if (take) buf[off + (unsigned int)__popcll(mask & below)] = (uint32_t)i;
+#pragma unroll
+            for (int u = 0; u < CULL_U; ++u) {
+                const int64_t i = i0 + kb + u * (int)blockDim.x + (int)threadIdx.x;
+                bool take = false;
+                if (i < A.n && kb + u * (int)blockDim.x < CULL_SPAN) {
+                    const double xh = (double)r2[u], yh = (double)r1[u], zh = (double)r0[u];
+                    const double xs = xh - A.chihview;
+                    const double chih = p2m_sqrt((xs * xs + yh * yh) + zh * zh);
+                    if (A.precut || !(chih > A.cut_high || chih < A.cut_low)) {
+                        tot_cut++;
+                        const double mass = mconst * cube((double)r6[u]);
+                        take = !(mass < A.mmin);
+                    }
+                }
+                const unsigned long long mask = __ballot(take);
+                if (mask) {
+                    unsigned int off = 0;
+                    if (lane == 0) off = atomicAdd(&nbuf, (unsigned int)__popcll(mask));
+                    off = __shfl(off, 0);
+                    if (take) buf[off + (unsigned int)__popcll(mask & below)] = (uint32_t)i;
+                }
             }
         }
         __syncthreads();
