@@ -130,6 +130,23 @@ __device__ static inline void block_count(unsigned long long *counter, unsigned 
     }
 }
 
+/* Inclusive prefix sum over the 64 lanes of a wave in registers: DPP row
+ * shifts by 1, 2, 4 and 8 scan each row of 16 lanes, then row broadcasts of
+ * lane 15 (into rows 1 and 3) and lane 31 (into rows 2 and 3) add the
+ * preceding rows' totals.  Lanes whose DPP source lies outside the row, and
+ * the rows a broadcast masks off, add 0.  Every lane of the wave must be
+ * active. */
+__device__ static inline uint32_t wave_incl_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   /* row_shr:1 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   /* row_shr:2 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   /* row_shr:4 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   /* row_shr:8 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   /* row_bcast:15 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   /* row_bcast:31 */
+    return v;
+}
+
 /* ===================================================================== */
 /* decode                                                                 */
 /* ===================================================================== */
@@ -1033,11 +1050,7 @@ __device__ static inline void tile_range(const TileArgs &A, int64_t &tbeg, int64
 __device__ static inline int batch_prepare(BatchLDS *bl, uint32_t len, int nstage, uint32_t &P)
 {
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
-    uint32_t v = len;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o);
-        if (lane >= o) v += u;
-    }
+    uint32_t v = wave_incl_scan(len);
     if (lane == 63) bl->wsum[wv] = v;
     lds_barrier();
     uint32_t add = 0;
@@ -1249,12 +1262,13 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
 struct FastSeg {
     uint32_t pre[FSEG + 1];        /* exclusive scan of the staged lengths */
     uint32_t seg_of[PMAX / 4];     /* batch pixel -> segment (bytes) */
-    uint32_t wsum[16];
-    uint32_t nb;
-    uint32_t nfb;
-    uint32_t lost;                 /* a deferred pixel of this tile found the list full */
+    uint32_t wsum[16];             /* per wave: staged lengths [0, 8), segments that fit [8, 16) */
+    /* the per-tile control words come in two slots, used by alternate
+     * tiles, so that a tile's end needs no barrier before the next begins */
+    uint32_t nfb[2];               /* deferred pixels of the tile */
+    uint32_t lost[2];              /* a deferred pixel of the tile found the list full */
+    int32_t next_tile[2];          /* the tile this workgroup paints next */
     unsigned long long fb_base;    /* list slot of the tile's deferred pixels */
-    int32_t next_tile;             /* the tile this workgroup paints next */
     uint4 fb[FB_LDS];
     uint32_t halo[FSEG];
     int16_t qoff[FSEG];            /* in-tile pixel minus batch pixel: lo - pre (-3072 .. 2047) */
@@ -1309,11 +1323,7 @@ __device__ static inline void load_floats(__amdgpu_buffer_rsrc_t r, int vo, int 
 __device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage, uint32_t &P)
 {
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
-    uint32_t v = len;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o);
-        if (lane >= o) v += u;
-    }
+    uint32_t v = wave_incl_scan(len);
     if (lane == 63) fs->wsum[wv] = v;
     lds_barrier();
     uint32_t add = 0;
@@ -1321,13 +1331,13 @@ __device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage,
     v += add;
     if (tid < nstage) fs->pre[tid + 1] = v;
     if (tid == 0) fs->pre[0] = 0;
+    /* the prefix is non-decreasing, so the leading segments that fit are
+     * the staged ones whose end is at most PMAX: count them per wave */
+    const unsigned long long fit = __ballot(tid < nstage && v <= PMAX);
+    if (lane == 0) fs->wsum[8 + wv] = (uint32_t)__popcll(fit);
     lds_barrier();
-    if (tid < nstage) {
-        const uint32_t e = fs->pre[tid + 1];
-        if (e <= PMAX && (tid + 1 == nstage || fs->pre[tid + 2] > PMAX)) fs->nb = (uint32_t)(tid + 1);
-    }
-    lds_barrier();
-    const int nb = (int)fs->nb;
+    int nb = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) nb += (int)fs->wsum[8 + k];
     P = fs->pre[nb];
     return nb;
 }
@@ -1390,7 +1400,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
     const int64_t fbs = A.fb_cap / P2M_SHARDS;
     unsigned long long *const fbc = A.fb_count + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE;
     uint4 *const fbr = A.fb + (blockIdx.x % P2M_SHARDS) * fbs;
-    if (tid == 0) { sl->nfb = 0; sl->lost = 0; }
+    if (tid == 0) { sl->nfb[0] = sl->nfb[1] = 0; sl->lost[0] = sl->lost[1] = 0; }
     for (int k = tid; k < nrt + 2; k += blockDim.x) invn[k] = A.inv_node2f[k];
     lds_barrier();
 
@@ -1399,27 +1409,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
      * (ring-major, so the tiles in flight on an XCD share halos in its L2)
      * from the XCD's counter.  A workgroup that becomes resident late, or
      * whose tiles run long, leaves the remaining tiles to the others.
-     * Thread 0 claims the following tile as a tile starts; the claim is
-     * published with the tile's last barrier. */
+     * Thread 0 claims the following tile with the tile's first segment
+     * loads and writes it to the other slot, which the tile's next barrier
+     * publishes.
+     *
+     * Tile k uses slot par = k & 1.  Its end needs no barrier: the next
+     * tile's first writes to shared words come after that tile's own
+     * barriers, its clear of the accumulator touches only the pixels each
+     * thread itself wrote back, and slot par ^ 1 is cleared by thread 0
+     * after tile k's post-loop barrier, when tile k-1's readers of it are
+     * done. */
     const bool xq = (gridDim.x % 8) == 0 && A.n_tiles >= 8;
     const int xcd = xq ? (int)(blockIdx.x % 8) : 0;
     const int64_t xb = xq ? (int64_t)xcd * A.n_tiles / 8 : 0;
     const int64_t xe = xq ? (int64_t)(xcd + 1) * A.n_tiles / 8 : A.n_tiles;
     unsigned int *const qctr = A.tile_next + xcd * P2M_TQ_STRIDE;
-    if (tid == 0) sl->next_tile = (int32_t)(xb + (int64_t)atomicAdd(qctr, 1u));
+    if (tid == 0) sl->next_tile[0] = (int32_t)(xb + (int64_t)atomicAdd(qctr, 1u));
     lds_barrier();
-    for (;;) {
+    for (int par = 0;; par ^= 1) {
         /* every thread reads the same tile: taken as uniform, its descriptor
          * and segment range are scalar loads */
-        const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane(sl->next_tile);
+        const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane(sl->next_tile[par]);
         if (t >= xe) break;
-        unsigned int claim = 0;
-        if (tid == 0) claim = atomicAdd(qctr, 1u);
+        bool claimed = false;
         const TileDesc G = tdesc[t];
         const uint64_t sbeg = toff[t], send = toff[t + 1];
         if (A.redo && !A.lost[t]) {      /* repaint pass: only the tiles that lost pixels */
             lds_barrier();
-            if (tid == 0) sl->next_tile = (int32_t)(xb + (int64_t)claim);
+            if (tid == 0) {
+                sl->nfb[par ^ 1] = 0; sl->lost[par ^ 1] = 0;
+                sl->next_tile[par ^ 1] = (int32_t)(xb + (int64_t)atomicAdd(qctr, 1u));
+            }
             lds_barrier();
             continue;
         }
@@ -1434,11 +1454,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             if (A.count) { lcount[q] = 0; lhash[q] = 0ULL; }
         }
         for (uint64_t b0 = sbeg; b0 < send;) {
-            lds_barrier();
+            /* a later batch restages what the previous one's pixels read
+             * (the first batch is ordered after the previous tile's reads by
+             * that tile's post-loop barrier) */
+            if (b0 != sbeg) lds_barrier();
             int nstage = (int)(send - b0 < FSEG ? send - b0 : FSEG);
             if (nstage > (int)blockDim.x) nstage = (int)blockDim.x;
             uint32_t len = 0;
-            const uint2 sg = tid < nstage ? A.segs[b0 + tid] : make_uint2(0u, 0u);
+            /* unconditional (the last staged entry for the idle lanes), so
+             * that its wait can move down to the first use */
+            const uint2 sg = A.segs[b0 + (uint64_t)(tid < nstage ? tid : nstage - 1)];
+            /* the claim of the following tile is made with the first
+             * batch's segment loads, so that one wait covers both */
+            if (!claimed) {
+                if (tid == 0) sl->next_tile[par ^ 1] = (int32_t)(xb + (int64_t)atomicAdd(qctr, 1u));
+                claimed = true;
+            }
             uint32_t slo = 0;
             if (tid < nstage) {
                 /* a segment must name a record and lie inside the tile; one
@@ -1453,19 +1484,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             const int nb = batch_prefix(sl, len, nstage, P);
             if (tid < nb) {
                 const TileHalo &h = A.hot[sg.x < (uint64_t)A.n_halo ? sg.x : 0u];
+                /* the record's fields, all requested before the first is used
+                 * (one memory round trip; the empty asm keeps them together) */
+                const double zz0 = h.zz0, s0 = h.s0, phi0 = h.phi0, thr = h.thr;
+                const float chi2 = h.chi2, fc = h.fc, fm = h.fm, vh = h.vh;
+                const int32_t colbase = h.colbase;
+                asm volatile("" :: "v"(zz0), "v"(s0), "v"(phi0), "v"(thr), "v"(chi2), "v"(fc), "v"(fm), "v"(vh), "v"(colbase));
                 sl->halo[tid] = sg.x;
                 sl->qoff[tid] = (int16_t)((int32_t)slo - (int32_t)sl->pre[tid]);
-                const double dz = G.zr - h.zz0, ds = G.sth - h.s0;
+                const double dz = G.zr - zz0, ds = G.sth - s0;
                 sl->A[tid] = (float)(dz * dz + ds * ds);
-                sl->B[tid] = (float)((4.0 * G.sth) * h.s0);
-                sl->thr_hi[tid] = (float)(h.thr * (1.0 + (double)A.band_c));
-                sl->chi2[tid] = h.chi2;
-                const float fc = h.fc, fm = h.fm, vh = h.vh;
+                sl->B[tid] = (float)((4.0 * G.sth) * s0);
+                sl->thr_hi[tid] = (float)(thr * (1.0 + (double)A.band_c));
+                sl->chi2[tid] = chi2;
                 sl->w00[tid] = vh * ((1.0f - fc) * (1.0f - fm));
                 sl->w10[tid] = vh * (fc * (1.0f - fm));
                 sl->w01[tid] = vh * ((1.0f - fc) * fm);
                 sl->w11[tid] = vh * (fc * fm);
-                sl->colbase[tid] = h.colbase;
+                sl->colbase[tid] = colbase;
                 /* azimuth origin: the ring pixel ip_c nearest the halo centre's
                  * azimuth, with x_c = ((ip_c + off) kphi - phi0) / 2 folded by
                  * pi (|x_c| <= kphi / 4) in FP64; a pixel's x is then x_c plus
@@ -1476,10 +1512,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                  * |x| <= pi/2 + pi/8 + kphi/4; on shorter rings each pixel's n
                  * is folded into [-npr/2, npr/2] as well.  Either way the FP32
                  * error of x stays relative to x. */
-                double ipc = __builtin_rint(h.phi0 * inv_kphi - G.off);
+                double ipc = __builtin_rint(phi0 * inv_kphi - G.off);
                 if (ipc < 0.0) ipc += (double)npr;
                 if (ipc >= (double)npr) ipc -= (double)npr;
-                double xc = 0.5 * ((ipc + G.off) * kphi - h.phi0);
+                double xc = 0.5 * ((ipc + G.off) * kphi - phi0);
                 xc -= HPX_PI * __builtin_rint(xc * (1.0 / HPX_PI));
                 sl->xc[tid] = (float)xc;
                 const int32_t nmid = G.ip0 + (int32_t)slo + (int32_t)(len >> 1) - (int32_t)ipc;
@@ -1496,19 +1532,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             auto paint_pixels = [&](auto fold) {
             for (uint32_t p = tid; p < P; p += blockDim.x) {
                 const int s = segb[p];
-                const int q = (int)p + sl->qoff[s];
-                int n = (int)p - sl->cidx[s];
+                /* every per-segment constant the accepted path needs is read
+                 * here, in one round of LDS reads: the empty asm below keeps
+                 * the compiler from moving the later ones behind the disc test,
+                 * where each would cost a round trip of its own */
+                const int cix = sl->cidx[s];
+                const float xcs = sl->xc[s], As = sl->A[s], Bs = sl->B[s], thr_hi = sl->thr_hi[s];
+                const float chi2 = sl->chi2[s];
+                const int qo = sl->qoff[s], cb = sl->colbase[s];
+                int n = (int)p - cix;
                 if constexpr (decltype(fold)::value) n = n > half_npr ? n - npr : (n < -half_npr ? n + npr : n);
-                const float x = __builtin_fmaf((float)n, hphi, sl->xc[s]);
+                const float x = __builtin_fmaf((float)n, hphi, xcs);
                 const float s2 = sin2f(x);
-                const float c2 = __builtin_fmaf(sl->B[s], s2, sl->A[s]);
-                const float thr_hi = sl->thr_hi[s];
+                const float c2 = __builtin_fmaf(Bs, s2, As);
+                asm volatile("" :: "v"(chi2), "v"(qo), "v"(cb));
                 if (!(c2 < thr_hi)) continue;                     /* outside the disc */
+                const int q = (int)p + qo;
                 /* tan^2(theta) = 4h(1-h)/(1-2h)^2 with h = chord^2/4 */
                 const float dd = __builtin_fmaf(c2, -0.5f, 1.0f);
                 const float rd = __builtin_amdgcn_rcpf(dd);
                 const float t2 = __builtin_fmaf(-0.25f * c2, c2, c2) * (rd * rd);
-                const float r2 = sl->chi2[s] * t2;
+                const float r2 = chi2 * t2;
                 const float u = __builtin_fmaf(__builtin_amdgcn_logf(r2), A.u_k1, -A.u_k0);
                 const bool zero = u > u_zero;                     /* r > rmaxt: value 0 */
                 /* deferred: near the disc edge, a degenerate chord, or u within
@@ -1517,13 +1561,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 const float ub = __builtin_amdgcn_fmed3f(u, 0.0f, A.u_max);
                 const bool defer = (c2 > thr_hi * lo_band) | !(dd > 1e-3f) | (__builtin_fabsf(u - __builtin_rintf(ub)) <= A.band);
                 if (defer) {
-                    const uint32_t slot = atomicAdd(&sl->nfb, 1u);
+                    const uint32_t slot = atomicAdd(&sl->nfb[par], 1u);
                     const uint4 e = make_uint4((uint32_t)G.ring, (uint32_t)(G.ip0 + q), sl->halo[s], 0u);
                     if (slot < FB_LDS) sl->fb[slot] = e;
                     else {
                         const unsigned long long gs = atomicAdd(fbc, 1ULL);
                         if (gs < (unsigned long long)fbs) fbr[gs] = e;
-                        else sl->lost = 1u;
+                        else sl->lost[par] = 1u;
                     }
                     tf++;
                     continue;
@@ -1541,7 +1585,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 const float fr = clampr ? A.fr_clamp : fr_in;
                 /* nodes ir-1 and ir of the cell: 4 corners of every map each,
                  * 32 NM contiguous bytes */
-                const int vo = (sl->colbase[s] + (ir - 1)) * (16 * NM);
+                const int vo = (cb + (ir - 1)) * (16 * NM);
                 const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
                 tu++;
                 float tq[8 * NM];
@@ -1570,18 +1614,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             if (npr < 4 * W) paint_pixels(std::true_type{});
             else paint_pixels(std::false_type{});
         }
+        if (!claimed && tid == 0)        /* a tile without segments */
+            sl->next_tile[par ^ 1] = (int32_t)(xb + (int64_t)atomicAdd(qctr, 1u));
         lds_barrier();
         /* the tile's deferred pixels still in LDS join the list before the
          * tile is written: a tile whose deferred pixels do not all fit in the
          * list writes nothing and is marked for the repaint pass */
-        const uint32_t nfl = sl->nfb < FB_LDS ? sl->nfb : FB_LDS;
+        const uint32_t nfb = sl->nfb[par];
+        const uint32_t nfl = nfb < FB_LDS ? nfb : FB_LDS;
         if (nfl) {
             if (tid == 0) sl->fb_base = atomicAdd(fbc, (unsigned long long)nfl);
             lds_barrier();
             for (uint32_t k = tid; k < nfl; k += blockDim.x)
                 if (sl->fb_base + k < (unsigned long long)fbs) fbr[sl->fb_base + k] = sl->fb[k];
         }
-        const bool lost = sl->lost != 0u || (nfl && sl->fb_base + nfl > (unsigned long long)fbs);
+        const bool lost = sl->lost[par] != 0u || (nfl && sl->fb_base + nfl > (unsigned long long)fbs);
         if (tid == 0) {
             if (lost) { A.lost[t] = 1; atomicAdd(A.n_lost + (blockIdx.x % P2M_SHARDS) * P2M_CTR_STRIDE, 1ULL); }
             else if (A.redo) A.lost[t] = 0;
@@ -1612,11 +1659,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 }
             }
         }
-        lds_barrier();
-        /* every thread has read nfb and lost; the reset and the next tile
-         * are published by the barrier below */
-        if (tid == 0) { sl->nfb = 0; sl->lost = 0; sl->next_tile = (int32_t)(xb + (int64_t)claim); }
-        lds_barrier();
+        if (tid == 0) { sl->nfb[par ^ 1] = 0; sl->lost[par ^ 1] = 0; }   /* the next tile's slot */
     }
     /* the staged constants are dead: their space takes the block sums (the
      * kernel has no static LDS, so its occupancy is what its launch asks) */
