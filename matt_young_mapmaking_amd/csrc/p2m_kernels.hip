@@ -1389,6 +1389,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
     /* inner edge of the deferral band, relative to the outer one */
     const float lo_band = (float)((1.0 - (double)A.band_c) / (1.0 + (double)A.band_c));
     const float u_zero = A.u_max + A.band;
+    /* constants of the pixel loop that an instruction needs in a vector
+     * register (a second scalar operand, a select), pinned there once */
+    float v_k0 = A.u_k0, v_frc = A.fr_clamp;
+    int v_irc = A.ir_clamp;
+    asm volatile("" : "+v"(v_k0), "+v"(v_frc), "+v"(v_irc));
 
     const int nrt = A.g.nrt;
     /* the corner-quad table as a raw buffer: a 32-bit byte offset per pixel,
@@ -1553,7 +1558,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 const float rd = __builtin_amdgcn_rcpf(dd);
                 const float t2 = __builtin_fmaf(-0.25f * c2, c2, c2) * (rd * rd);
                 const float r2 = chi2 * t2;
-                const float u = __builtin_fmaf(__builtin_amdgcn_logf(r2), A.u_k1, -A.u_k0);
+                const float u = __builtin_fmaf(__builtin_amdgcn_logf(r2), A.u_k1, -v_k0);
                 const bool zero = u > u_zero;                     /* r > rmaxt: value 0 */
                 /* deferred: near the disc edge, a degenerate chord, or u within
                  * the band of a cell boundary 0..u_max (u clamped to that range,
@@ -1577,19 +1582,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                  * truncation is floor); a clamped r (r = rmint + 1e-5) has the cell
                  * and fraction the host computed for it */
                 const int ir_u = min(max((int)u + 1, 1), nrt - 1);
-                const int ir = clampr ? A.ir_clamp : ir_u;
+                const int ir = clampr ? v_irc : ir_u;
+                /* nodes ir-1 and ir of the cell: 4 corners of every map each,
+                 * 32 NM contiguous bytes, requested before the radial fraction
+                 * so that their latency covers its LDS read and log */
+                const int vo = (cb + (ir - 1)) * (16 * NM);
+                float tq[8 * NM];
+                load_floats<8 * NM>(trs, vo, 0, tq);
+                const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
                 /* fr = ln(r / r_node) = (ln 2 / 2) log2(r^2 / r_node^2): the ratio is
                  * in [1, 1.08), where the hardware log2 is within one ulp
                  * (7e-9); both sides evaluated, then selected */
-                const float fr_in = 0.34657359f * __builtin_amdgcn_logf(r2 * invn[ir - 1]);
-                const float fr = clampr ? A.fr_clamp : fr_in;
-                /* nodes ir-1 and ir of the cell: 4 corners of every map each,
-                 * 32 NM contiguous bytes */
-                const int vo = (cb + (ir - 1)) * (16 * NM);
-                const float w00 = sl->w00[s], w10 = sl->w10[s], w01 = sl->w01[s], w11 = sl->w11[s];
+                float fr_in = 0.34657359f * __builtin_amdgcn_logf(r2 * invn[ir - 1]);
+                asm volatile("" : "+v"(fr_in));   /* computed on every lane: a select, not a branch */
+                const float fr = clampr ? v_frc : fr_in;
                 tu++;
-                float tq[8 * NM];
-                load_floats<8 * NM>(trs, vo, 0, tq);
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
                     const float *c0v = tq + 4 * m, *c1v = tq + 4 * (NM + m);
