@@ -1468,7 +1468,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             uint32_t len = 0;
             /* unconditional (the last staged entry for the idle lanes), so
              * that its wait can move down to the first use */
-            const uint2 sg = A.segs[b0 + (uint64_t)(tid < nstage ? tid : nstage - 1)];
+            /* a streaming load: the segment list is read once */
+            const unsigned long long sgw = __builtin_nontemporal_load(
+                reinterpret_cast<const unsigned long long *>(A.segs) + b0 + (uint64_t)(tid < nstage ? tid : nstage - 1));
+            const uint2 sg = make_uint2((uint32_t)sgw, (uint32_t)(sgw >> 32));
             /* the claim of the following tile is made with the first
              * batch's segment loads, so that one wait covers both */
             if (!claimed) {
@@ -1659,7 +1662,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
             }
         } else {
             for (int q = tid; q < G.width; q += blockDim.x) {
-                for (int m = 0; m < NM; ++m) A.map32[m][pix0 + q] = (float)acc[m * W + q];
+                /* streaming stores: the map is not read again in the pass, so
+                 * it should not displace halo records and table rows in L2 */
+                for (int m = 0; m < NM; ++m) __builtin_nontemporal_store((float)acc[m * W + q], A.map32[m] + pix0 + q);
                 if (A.count) {
                     A.count[pix0 + q] = lcount[q];
                     A.hash[pix0 + q] = lhash[q];
