@@ -1320,7 +1320,7 @@ __device__ static inline void load_floats(__amdgpu_buffer_rsrc_t r, int vo, int 
  * pixels fit in PMAX (a segment is at most W <= PMAX pixels, so at least
  * one).  Returns their number; P receives their pixel count.  Every thread
  * calls; ends with a barrier. */
-__device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage, uint32_t &P)
+__device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage, uint32_t &P, uint32_t &start)
 {
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     uint32_t v = wave_incl_scan(len);
@@ -1329,6 +1329,7 @@ __device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage,
     uint32_t add = 0;
     for (int k = 0; k < wv; ++k) add += fs->wsum[k];
     v += add;
+    start = v - len;                 /* this thread's segment starts at batch pixel pre[tid] */
     if (tid < nstage) fs->pre[tid + 1] = v;
     if (tid == 0) fs->pre[0] = 0;
     /* the prefix is non-decreasing, so the leading segments that fit are
@@ -1342,33 +1343,47 @@ __device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage,
     return nb;
 }
 
-/* The batch's pixel -> segment byte map: each thread takes a run of whole
- * words, finds the segment of its first pixel by binary search over pre and
- * walks forward.  The caller's barrier publishes it. */
-__device__ static inline void build_seg_map(FastSeg *fs, int nb, uint32_t P)
+/* The batch's pixel -> segment byte map, written by the segments
+ * themselves: thread s (< nb) owns pixels [a, a + len).  A segment of at
+ * most 64 pixels is written by its own thread (bytes up to a word boundary,
+ * whole words, the tail bytes); a longer one by its whole wave, 64 words a
+ * round, its head and tail bytes by six lanes.  Every byte belongs to one
+ * segment and whole-word writes only cover words inside one, so the writes
+ * never overlap; there are no reads.  Every lane of the wave must be
+ * active.  The caller's barrier publishes the map. */
+__device__ static inline void build_seg_map_direct(FastSeg *fs, int nb, uint32_t a, uint32_t len)
 {
-    const uint32_t nw = (P + 3) >> 2;
-    const uint32_t wpt = (nw + blockDim.x - 1) / blockDim.x;
-    const uint32_t w0 = threadIdx.x * wpt;
-    const uint32_t w1 = w0 + wpt < nw ? w0 + wpt : nw;
-    if (w0 >= w1) return;
-    const uint32_t p0 = w0 * 4;
-    int lo_s = 0, hi_s = nb - 1;
-    while (lo_s < hi_s) {
-        const int mid = (lo_s + hi_s + 1) >> 1;
-        if (fs->pre[mid] <= p0) lo_s = mid; else hi_s = mid - 1;
+    unsigned char *mb = reinterpret_cast<unsigned char *>(fs->seg_of);
+    uint32_t *mw = fs->seg_of;
+    const int tid = threadIdx.x, lane = __lane_id();
+    const bool mine = tid < nb && len > 0;
+    const bool lng = mine && len > 64;
+    if (mine && !lng) {
+        const uint32_t e = a + len, fill = (uint32_t)tid * 0x01010101u;
+        uint32_t p = a;
+        for (; p < e && (p & 3u); ++p) mb[p] = (unsigned char)tid;
+        for (; p + 4 <= e; p += 4) mw[p >> 2] = fill;
+        for (; p < e; ++p) mb[p] = (unsigned char)tid;
     }
-    uint32_t sg = (uint32_t)lo_s, nxt = fs->pre[sg + 1];
-    for (uint32_t w = w0; w < w1; ++w) {
-        uint32_t o = 0;
-        for (int bb = 0; bb < 4; ++bb) {
-            const uint32_t p = w * 4 + bb;
-            while (p >= nxt && (int)sg + 1 < nb) { ++sg; nxt = fs->pre[sg + 1]; }
-            o |= sg << (8 * bb);
+    unsigned long long lm = __ballot(lng);
+    while (lm) {
+        const int L = __ffsll((long long)lm) - 1;
+        lm &= lm - 1;
+        const uint32_t aL = (uint32_t)__builtin_amdgcn_readlane((int)a, L);
+        const uint32_t eL = aL + (uint32_t)__builtin_amdgcn_readlane((int)len, L);
+        const uint32_t sL = (uint32_t)((tid & ~63) + L), fL = sL * 0x01010101u;
+        const uint32_t w0 = (aL + 3u) >> 2, w1 = eL >> 2;     /* whole words [w0, w1): len > 64, so w0 < w1 */
+        for (uint32_t w = w0 + (uint32_t)lane; w < w1; w += 64) mw[w] = fL;
+        if (lane < 3) {
+            const uint32_t p = aL + (uint32_t)lane;             /* head bytes below 4 w0 */
+            if (p < 4u * w0) mb[p] = (unsigned char)sL;
+        } else if (lane < 6) {
+            const uint32_t p = 4u * w1 + (uint32_t)(lane - 3);   /* tail bytes from 4 w1 */
+            if (p < eL) mb[p] = (unsigned char)sL;
         }
-        fs->seg_of[w] = o;
     }
 }
+
 
 template <int NM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 1 ? 6 : 1))) k_tile_fast(TileArgs A, const TileDesc *__restrict__ tdesc,
@@ -1489,7 +1504,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 len = ok ? slen : 0u;
             }
             uint32_t P;
-            const int nb = batch_prefix(sl, len, nstage, P);
+            uint32_t start;
+            const int nb = batch_prefix(sl, len, nstage, P, start);
             if (tid < nb) {
                 const TileHalo &h = A.hot[sg.x < (uint64_t)A.n_halo ? sg.x : 0u];
                 /* the record's fields, all requested before the first is used
@@ -1499,7 +1515,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 const int32_t colbase = h.colbase;
                 asm volatile("" :: "v"(zz0), "v"(s0), "v"(phi0), "v"(thr), "v"(chi2), "v"(fc), "v"(fm), "v"(vh), "v"(colbase));
                 sl->halo[tid] = sg.x;
-                sl->qoff[tid] = (int16_t)((int32_t)slo - (int32_t)sl->pre[tid]);
+                sl->qoff[tid] = (int16_t)((int32_t)slo - (int32_t)start);
                 const double dz = G.zr - zz0, ds = G.sth - s0;
                 sl->A[tid] = (float)(dz * dz + ds * ds);
                 sl->B[tid] = (float)((4.0 * G.sth) * s0);
@@ -1527,11 +1543,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 xc -= HPX_PI * __builtin_rint(xc * (1.0 / HPX_PI));
                 sl->xc[tid] = (float)xc;
                 const int32_t nmid = G.ip0 + (int32_t)slo + (int32_t)(len >> 1) - (int32_t)ipc;
-                sl->cidx[tid] = (int32_t)ipc - G.ip0 - ((int32_t)slo - (int32_t)sl->pre[tid]) +
+                sl->cidx[tid] = (int32_t)ipc - G.ip0 - ((int32_t)slo - (int32_t)start) +
                                 (nmid > half_npr ? npr : (nmid < -half_npr ? -npr : 0));
                 if (A.count) hh[tid] = h.hash;
             }
-            build_seg_map(sl, nb, P);
+            build_seg_map_direct(sl, nb, start, len);
             lds_barrier();
             b0 += (uint64_t)nb;
             const unsigned char *segb = (const unsigned char *)sl->seg_of;
