@@ -990,8 +990,13 @@ int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *
  * latency of the map stores at every tile boundary.) */
 __device__ static inline void lds_barrier()
 {
+    /* LDS-only release and acquire fences around the barrier: the compiler
+     * keeps LDS accesses on their side of it, and the hardware waits only
+     * for LDS traffic (lgkmcnt), not for outstanding global memory */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_waitcnt(0xc07f);      /* lgkmcnt(0); vmcnt, expcnt unconstrained */
     __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 #define FSEG 256                   /* segments staged per batch */
@@ -1260,7 +1265,7 @@ __global__ void __launch_bounds__(512) k_tile_exact(TileArgs A)
  * broadcast in one pass, where 16-byte reads of a struct keep the LDS unit
  * several times busier. */
 struct FastSeg {
-    uint32_t pre[FSEG + 1];        /* exclusive scan of the staged lengths */
+    uint32_t nb_s, P_s;            /* the batch's segment and pixel counts (set between its barriers) */
     uint32_t seg_of[PMAX / 4];     /* batch pixel -> segment (bytes) */
     uint32_t wsum[16];             /* per wave: staged lengths [0, 8), segments that fit [8, 16) */
     /* the per-tile control words come in two slots, used by alternate
@@ -1316,47 +1321,48 @@ __device__ static inline void load_floats(__amdgpu_buffer_rsrc_t r, int vo, int 
     for (; k < NF; ++k) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo + 4 * k, so, 0));
 }
 
-/* Scan of the staged lengths into fs->pre, and the leading segments whose
- * pixels fit in PMAX (a segment is at most W <= PMAX pixels, so at least
- * one).  Returns their number; P receives their pixel count.  Every thread
- * calls; ends with a barrier. */
-__device__ static inline int batch_prefix(FastSeg *fs, uint32_t len, int nstage, uint32_t &P, uint32_t &start)
+/* Scan of the staged lengths: start receives this thread's first batch
+ * pixel and fit whether its segment belongs to the batch, i.e. lies within
+ * the first PMAX pixels (a segment is at most W <= PMAX pixels, so at least
+ * one does; the prefix is non-decreasing, so the batch is a leading run of
+ * the staged segments).  The batch's segment and pixel counts reach
+ * fs->nb_s and fs->P_s through LDS atomics, readable after the caller's next
+ * barrier.  Every thread calls; one barrier. */
+__device__ static inline void batch_prefix(FastSeg *fs, uint32_t len, int nstage, uint32_t &start, bool &fit)
 {
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     uint32_t v = wave_incl_scan(len);
     if (lane == 63) fs->wsum[wv] = v;
+    if (tid == 0) { fs->nb_s = 0; fs->P_s = 0; }
     lds_barrier();
     uint32_t add = 0;
     for (int k = 0; k < wv; ++k) add += fs->wsum[k];
     v += add;
-    start = v - len;                 /* this thread's segment starts at batch pixel pre[tid] */
-    if (tid < nstage) fs->pre[tid + 1] = v;
-    if (tid == 0) fs->pre[0] = 0;
-    /* the prefix is non-decreasing, so the leading segments that fit are
-     * the staged ones whose end is at most PMAX: count them per wave */
-    const unsigned long long fit = __ballot(tid < nstage && v <= PMAX);
-    if (lane == 0) fs->wsum[8 + wv] = (uint32_t)__popcll(fit);
-    lds_barrier();
-    int nb = 0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) nb += (int)fs->wsum[8 + k];
-    P = fs->pre[nb];
-    return nb;
+    start = v - len;
+    fit = tid < nstage && v <= PMAX;
+    const unsigned long long fm = __ballot(fit);
+    if (fm) {
+        /* the wave's fitting lanes are lanes 0 .. last */
+        const int last = 63 - __clzll((long long)fm);
+        if (lane == 0) atomicAdd(&fs->nb_s, (uint32_t)__popcll(fm));
+        if (lane == last) atomicMax(&fs->P_s, v);
+    }
 }
 
 /* The batch's pixel -> segment byte map, written by the segments
- * themselves: thread s (< nb) owns pixels [a, a + len).  A segment of at
+ * themselves: thread s (in the batch) owns pixels [a, a + len).  A segment of at
  * most 64 pixels is written by its own thread (bytes up to a word boundary,
  * whole words, the tail bytes); a longer one by its whole wave, 64 words a
  * round, its head and tail bytes by six lanes.  Every byte belongs to one
  * segment and whole-word writes only cover words inside one, so the writes
  * never overlap; there are no reads.  Every lane of the wave must be
  * active.  The caller's barrier publishes the map. */
-__device__ static inline void build_seg_map_direct(FastSeg *fs, int nb, uint32_t a, uint32_t len)
+__device__ static inline void build_seg_map_direct(FastSeg *fs, bool in_batch, uint32_t a, uint32_t len)
 {
     unsigned char *mb = reinterpret_cast<unsigned char *>(fs->seg_of);
     uint32_t *mw = fs->seg_of;
     const int tid = threadIdx.x, lane = __lane_id();
-    const bool mine = tid < nb && len > 0;
+    const bool mine = in_batch && len > 0;
     const bool lng = mine && len > 64;
     if (mine && !lng) {
         const uint32_t e = a + len, fill = (uint32_t)tid * 0x01010101u;
@@ -1503,10 +1509,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                 if (!ok) n_err++;
                 len = ok ? slen : 0u;
             }
-            uint32_t P;
             uint32_t start;
-            const int nb = batch_prefix(sl, len, nstage, P, start);
-            if (tid < nb) {
+            bool fit;
+            batch_prefix(sl, len, nstage, start, fit);
+            if (fit) {
                 const TileHalo &h = A.hot[sg.x < (uint64_t)A.n_halo ? sg.x : 0u];
                 /* the record's fields, all requested before the first is used
                  * (one memory round trip; the empty asm keeps them together) */
@@ -1547,8 +1553,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NM == 
                                 (nmid > half_npr ? npr : (nmid < -half_npr ? -npr : 0));
                 if (A.count) hh[tid] = h.hash;
             }
-            build_seg_map_direct(sl, nb, start, len);
+            build_seg_map_direct(sl, fit, start, len);
             lds_barrier();
+            const int nb = (int)sl->nb_s;
+            const uint32_t P = sl->P_s;
             b0 += (uint64_t)nb;
             const unsigned char *segb = (const unsigned char *)sl->seg_of;
             /* the pixel loop, with or without the per-pixel fold of n (a
