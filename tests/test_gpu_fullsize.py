@@ -59,7 +59,7 @@ def test_fullsize_fp32_against_fp64(wl, masses):
     del h
     freqs = TABLES_C5 if wl == "C5" else [(148.0, 0.0)]
     tabs = [abi.make_maptable(model=1, nu=nu, alpha=a) for nu, a in freqs]
-    tile = (1024, 256) if len(tabs) == 1 else (512, 256)
+    tile = (1024, 256)        # the widest shape that keeps two six-map workgroups per CU
     member = wl == "C4"
     chunk = (1 << 24) if wl == "C4" else 0       # C4 is defined as 2^24-halo batches
     # the FP32 maps go to the host before the FP64 context is made: at C5 each
