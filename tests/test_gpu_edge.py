@@ -217,3 +217,33 @@ def test_clamped_radius_cell_of_a_fine_radial_grid():
     nz = ref["hp"] != 0
     assert nz.sum() > 1000
     assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_COMP])
+def test_dense_tiles_take_several_batches(table, prec):
+    """4000 discs clustered within about 0.03 rad of one equatorial direction
+    at nside 512 (up to 1200 halos on one pixel): the tiles under the cluster
+    list up to 2e4 pixels, far more than one batch holds (3072), and hundreds
+    of segments, so the painters stage them in several batches per tile
+    (later batches behind their own barrier, the segment-written byte map
+    per batch, the batch counts formed anew).  Membership and values against
+    the oracle."""
+    print(f"\n[gpu edge] dense tiles precision={prec}", flush=True)
+    rng = np.random.default_rng(2468)
+    n = 4000
+    h = np.zeros((n, 7), np.float32)
+    c0 = 250.0 / np.sqrt(2.0)                            # chi = 250 Mpc, beyond the z = 0.05 cut
+    h[:, 0] = rng.uniform(-5.0, 5.0, n)                  # hpvec z = pos[0] / chi: near the equator
+    h[:, 1] = c0 + rng.uniform(-5.0, 5.0, n)
+    h[:, 2] = c0 + rng.uniform(-5.0, 5.0, n)
+    h[:, 6] = rng.uniform(5.0, 9.0, n)                   # Lagrangian radius in Mpc
+    ref, hp, fs, (hc, hh, fc, fh), st = paint_both(h, table, nside=512, npix=64, prec=prec)
+    # in the equatorial belt every ring starts on a multiple of 1024 and holds
+    # 2048 pixels, so the 1024-pixel tiles are the aligned 1024-pixel windows
+    per_tile = ref["hp_count"].astype(np.int64).reshape(-1, 1024).sum(axis=1)
+    assert per_tile.max() > 3 * 3072, per_tile.max()
+    assert np.array_equal(hc, ref["hp_count"]) and np.array_equal(hh, ref["hp_hash"])
+    nz = ref["hp"] != 0
+    assert np.array_equal(hp != 0, nz)
+    tol = 1e-6 if prec == abi.PREC_FP32_COMP else 1e-12
+    assert np.max(np.abs(hp[nz] / ref["hp"][nz] - 1)) <= tol
