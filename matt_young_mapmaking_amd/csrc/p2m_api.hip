@@ -771,7 +771,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             const uint32_t key_max = p2m_key_max(c->cfg.nside, std::max(1, c->cfg.nside / 128));
             size_t need = 0;
             if (p2m_sort_halos(c->d_halo, n_halo, key_max, c->d_bin, c->d_hot, c->d_halo_s, c->g.nrt, c->g.nmht,
-                               c->d_keys, c->d_vals, nullptr, &need, c->stream)) return P2M_E_HIP;
+                               c->d_keys, c->d_vals, nullptr, &need, c->d_nr, c->stream)) return P2M_E_HIP;
             if (need > c->sort_tmp_bytes || !c->d_sort_tmp) {
                 dfree(c->d_sort_tmp); c->d_sort_tmp = nullptr; c->sort_tmp_bytes = 0;
                 if (dmalloc(&c->d_sort_tmp, need + 256)) return P2M_E_NOMEM;
@@ -779,7 +779,7 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
             }
             size_t bytes = c->sort_tmp_bytes;
             if (p2m_sort_halos(c->d_halo, n_halo, key_max, c->d_bin, c->d_hot, c->d_halo_s, c->g.nrt, c->g.nmht,
-                               c->d_keys, c->d_vals, c->d_sort_tmp, &bytes, c->stream)) return P2M_E_HIP;
+                               c->d_keys, c->d_vals, c->d_sort_tmp, &bytes, c->d_nr, c->stream)) return P2M_E_HIP;
         }
         const uint32_t *order = c->d_vals + n_halo;   /* spatial position -> slot */
         BinArgs b;
@@ -788,8 +788,9 @@ static int paint_halos(p2m_ctx *c, int64_t n_halo, int64_t n_flat, unsigned long
         b.nr = c->d_nr; b.ring_off = c->d_ring_off;
         b.tile_base = c->d_tile_base; b.tile_cnt = c->d_tile_cnt;
         b.tile_off = c->d_tile_off; b.segs = c->d_segs; b.counters = pc; b.ring_z = c->d_ring_z;
-        /* (halo, ring) item space */
-        p2m_launch_ring_counts(b, c->stream);
+        /* (halo, ring) item space: the gather wrote the item count of every
+         * halo (with no halo it did not run) */
+        if (n_halo <= 0) p2m_launch_ring_counts(b, c->stream);
         { int rc = scan_counts(c, c->d_nr, c->d_ring_off, n_halo + 1); if (rc) return rc; }
         /* n_items (= ring_off[n_halo]) was counted by the decode */
         if ((int64_t)n_items > c->runs_cap || !c->d_runs || !c->d_pos) {

@@ -175,7 +175,7 @@ void p2m_launch_seg_emit(const BinArgs &b, hipStream_t s);
 uint32_t p2m_key_max(int32_t nside, int32_t nphi);
 int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin, TileHalo *hot, HaloRec *full,
                    int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
-                   hipStream_t s);
+                   uint32_t *nr, hipStream_t s);
 int p2m_scan_u32(const uint32_t *cnt, unsigned long long *off, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t s);
 int p2m_tile_max_log2w(void);
 void p2m_launch_build_quad(const float *slice, int64_t n_slice, int32_t nrt, int32_t nmht, int32_t nchit,

@@ -878,8 +878,13 @@ static_assert(offsetof(HaloRec, zz0) == 32 && offsetof(HaloRec, phi0) == 40 && o
               offsetof(HaloRec, chi_out) == 152 && sizeof(HaloRec) == 176, "HaloRec layout read by k_gather_halos");
 
 __global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const uint32_t *order, int64_t n,
-                                                      BinHalo *bin, TileHalo *hot, HaloRec *full, int32_t nrt, int32_t nmht)
+                                                      BinHalo *bin, TileHalo *hot, HaloRec *full, int32_t nrt, int32_t nmht,
+                                                      uint32_t *nr)
 {
+    /* nr: the (halo, ring) items of every halo in spatial order (its ring
+     * range, or one item for an inert record), and 0 after the last, as
+     * the scan into ring offsets needs */
+    if (blockIdx.x == 0 && threadIdx.x == 0) nr[n] = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t o = order[i];
         const uint4 *src = reinterpret_cast<const uint4 *>(in + o);
@@ -896,6 +901,7 @@ __global__ void __launch_bounds__(256) k_gather_halos(const HaloRec *in, const u
         b.cosang = cosang; b.zz0 = zz0; b.a = a; b.phi0 = phi0;
         b.irmin = irmin; b.irmax = irmax;
         bin[i] = b;
+        nr[i] = irmax >= irmin ? (uint32_t)(irmax - irmin + 1) : 1u;
         if (hot) {
             TileHalo t;
             t.zz0 = zz0; t.s0 = s0; t.phi0 = phi0; t.thr = thr;
@@ -927,7 +933,7 @@ uint32_t p2m_key_max(int32_t nside, int32_t nphi)
  * in that order.  keys and vals hold 2n entries: input then output. */
 int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin, TileHalo *hot, HaloRec *full,
                    int32_t nrt, int32_t nmht, uint32_t *keys, uint32_t *vals, void *tmp, size_t *tmp_bytes,
-                   hipStream_t s)
+                   uint32_t *nr, hipStream_t s)
 {
     int bits = 1;
     while ((1u << bits) <= key_max && bits < 32) ++bits;
@@ -941,7 +947,7 @@ int p2m_sort_halos(const HaloRec *in, int64_t n, uint32_t key_max, BinHalo *bin,
                                                       (int)n, 0, bits, s);
     if (e != hipSuccess) return -1;
     unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_gather_halos, dim3(g), dim3(256), 0, s, in, vals + n, n, bin, hot, full, nrt, nmht);
+    hipLaunchKernelGGL(k_gather_halos, dim3(g), dim3(256), 0, s, in, vals + n, n, bin, hot, full, nrt, nmht, nr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
