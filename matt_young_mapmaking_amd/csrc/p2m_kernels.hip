@@ -6,8 +6,10 @@
  *                    (pks2map.f90:144-160, 264, 284-298; haloproject.f90:47-55,
  *                    93-128; maptable.f90:184-202; query_disc prologue)
  *  k_gather_halos    the survivors in spatial order (band of rings x azimuth),
- *                    packed for the binning and the FP32 painter
- *  k_ring_counts, k_group_heads, k_seg_count, k_seg_emit
+ *                    packed for the binning and the FP32 painter, with each
+ *                    halo's (halo, ring) item count
+ *  k_ring_counts (a pass with no painted halo), k_group_heads, k_seg_count,
+ *  k_seg_emit
  *                    bin the (halo, ring) pixel runs of query_disc into ring
  *                    tiles of W pixels (pix_tools.F90:712-800,
  *                    pixel_routines.F90:1823-1879)
